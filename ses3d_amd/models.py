@@ -1,0 +1,305 @@
+"""Synthetic model and input generator for SES3D runs.
+
+Produces the model fields the reference reads (``rhoinv, mu, lambda, A, B, C``
+and ``Q``, SOURCE/ses3d_input.f90:472-501) on the GLL points of a spherical
+section, and writes a complete run tree (``INPUT/``, ``MODELS/MODELS/``,
+``ADJOINT/``, ``DATA/``, ``MAIN/``) in the reference formats.
+
+The roles it plays follow MODELS/SOURCE/generate_models.f90 (boxfile and model
+files, :200-452), MODELS/MODELS_1D/prem_iso.f90 (a 1D isotropic earth) and
+MODELS/SOURCE/add_perturbation.f90 (3D perturbations of vp/vs/rho).  The 1D
+profile here is a simplified piecewise-linear PREM-like model, not PREM's
+polynomial coefficients; the generator only needs to produce inputs that both
+the reference and this solver consume identically.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field, replace
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import formats as F
+
+# GLL knots for lpd=4 (closed form: 0, +-sqrt(3/7), +-1); SOURCE/ses3d_init.f90:127-139
+KNOTS64 = np.array([-1.0, -np.sqrt(3.0 / 7.0), 0.0, np.sqrt(3.0 / 7.0), 1.0])
+
+
+def gll_coords(box: F.Box):
+    """Physical coordinates of the GLL points of one rank box (float64).
+
+    x(i,n) = xmin + i dx + (1+knot_n) dx/2 (colatitude, rad);
+    y likewise (longitude); z(k,n) = zmax - k dz - (1+knot_n) dz/2 (radius, m),
+    i.e. radius decreases with k as in SOURCE/ses3d_init.f90:222-242.
+    """
+    nx, ny, nz = box.nx, box.ny, box.nz
+    dx = (box.xmax - box.xmin) / (nx + 1)
+    dy = (box.ymax - box.ymin) / (ny + 1)
+    dz = (box.zmax - box.zmin) / (nz + 1)
+    i = np.arange(nx + 1)[:, None]
+    j = np.arange(ny + 1)[:, None]
+    k = np.arange(nz + 1)[:, None]
+    x = box.xmin + i * dx + 0.5 * (1 + KNOTS64[None, :]) * dx
+    y = box.ymin + j * dy + 0.5 * (1 + KNOTS64[None, :]) * dy
+    z = box.zmax - k * dz - 0.5 * (1 + KNOTS64[None, :]) * dz
+    return x, y, z
+
+
+def _broadcast(box: F.Box, x, y, z):
+    """Broadcast per-axis arrays to the reference layout (nx+1,ny+1,nz+1,5,5,5)."""
+    X = x[:, None, None, :, None, None]
+    Y = y[None, :, None, None, :, None]
+    Z = z[None, None, :, None, None, :]
+    return np.broadcast_arrays(X, Y, Z)
+
+
+# ---------------------------------------------------------------------------
+# 1D earth
+# ---------------------------------------------------------------------------
+
+EARTH_R = 6371000.0
+# (radius [m], rho [kg/m^3], vp [m/s], vs [m/s]) nodes of a piecewise-linear
+# PREM-like isotropic profile for the upper 1000 km.
+_PROFILE = np.array([
+    [5371000.0, 4380.0, 10750.0, 5950.0],
+    [5701000.0, 3990.0, 10200.0, 5570.0],
+    [5711000.0, 3920.0, 9100.0, 4930.0],
+    [5961000.0, 3540.0, 8560.0, 4640.0],
+    [5971000.0, 3540.0, 8200.0, 4500.0],
+    [6151000.0, 3370.0, 8030.0, 4450.0],
+    [6346000.0, 3380.0, 8100.0, 4490.0],
+    [6356000.0, 2900.0, 6800.0, 3900.0],
+    [6371000.0, 2600.0, 5800.0, 3200.0],
+])
+
+
+def prem_like(r: np.ndarray):
+    """-> rho, vp, vs (float64) at radius r [m], piecewise linear in r."""
+    r = np.asarray(r, np.float64)
+    rr = _PROFILE[:, 0]
+    return (np.interp(r, rr, _PROFILE[:, 1]), np.interp(r, rr, _PROFILE[:, 2]),
+            np.interp(r, rr, _PROFILE[:, 3]))
+
+
+def q_mu_by_depth(r: np.ndarray) -> np.ndarray:
+    """Shear Q between 80 (low-velocity zone) and 600 (crust/deep), by depth."""
+    depth = EARTH_R - np.asarray(r, np.float64)
+    d = np.array([0.0, 24e3, 80e3, 220e3, 400e3, 670e3, 1000e3])
+    q = np.array([600.0, 600.0, 80.0, 80.0, 143.0, 312.0, 312.0])
+    return np.interp(depth, d, q)
+
+
+# ---------------------------------------------------------------------------
+# 3D perturbations
+# ---------------------------------------------------------------------------
+
+def _cartesian(colat, lon, r):
+    st = np.sin(colat)
+    return r * st * np.cos(lon), r * st * np.sin(lon), r * np.cos(colat)
+
+
+def gaussian_random_field(box: F.Box, seed: int, corr_len: float = 200e3, rms: float = 0.03,
+                          spacing: float = 40e3):
+    """Seeded Gaussian-correlated random field on the box's GLL points.
+
+    White noise on a regular grid in (arc-length θ, arc-length φ, depth) with
+    ``spacing`` metres, filtered by a Gaussian kernel of correlation length
+    ``corr_len`` in Fourier space, scaled to ``rms``, then interpolated
+    trilinearly to every GLL point.  Returns an array in reference layout.
+    """
+    from scipy.interpolate import RegularGridInterpolator
+
+    rng = np.random.default_rng(seed)
+    lx = (box.xmax - box.xmin) * EARTH_R
+    ly = (box.ymax - box.ymin) * EARTH_R * np.sin(0.5 * (box.xmin + box.xmax))
+    lz = box.zmax - box.zmin
+    pad = 2 * corr_len
+    n = [max(4, int(np.ceil((L + 2 * pad) / spacing)) + 1) for L in (lx, ly, lz)]
+    noise = rng.standard_normal(n)
+    k = [np.fft.fftfreq(m, d=spacing) * 2 * np.pi for m in n]
+    K2 = k[0][:, None, None] ** 2 + k[1][None, :, None] ** 2 + k[2][None, None, :] ** 2
+    filt = np.exp(-0.25 * K2 * corr_len ** 2)
+    f = np.real(np.fft.ifftn(np.fft.fftn(noise) * filt))
+    f *= rms / max(f.std(), 1e-30)
+    ax = [np.arange(m) * spacing - pad for m in n]
+    interp = RegularGridInterpolator(ax, f, bounds_error=False, fill_value=0.0)
+    x, y, z = gll_coords(box)
+    X, Y, Z = _broadcast(box, x, y, z)
+    sx = (X - box.xmin) * EARTH_R
+    sy = (Y - box.ymin) * EARTH_R * np.sin(0.5 * (box.xmin + box.xmax))
+    sz = box.zmax - Z
+    pts = np.stack([sx.ravel(order="F"), sy.ravel(order="F"), sz.ravel(order="F")], axis=1)
+    return interp(pts).reshape(X.shape, order="F")
+
+
+def gaussian_blobs(box: F.Box, seed: int, n_blobs: int = 6, radius: float = 300e3, amp: float = 0.02):
+    """Sum of ``n_blobs`` seeded Gaussian blobs of the given radius and peak
+    amplitude (random sign), centred inside the section."""
+    rng = np.random.default_rng(seed)
+    x, y, z = gll_coords(box)
+    X, Y, Z = _broadcast(box, x, y, z)
+    P = np.stack(_cartesian(X, Y, Z))
+    out = np.zeros(X.shape)
+    for _ in range(n_blobs):
+        c = (rng.uniform(box.xmin, box.xmax), rng.uniform(box.ymin, box.ymax),
+             rng.uniform(box.zmin + 0.2 * (box.zmax - box.zmin), box.zmax - 0.1 * (box.zmax - box.zmin)))
+        C = np.array(_cartesian(*c)).reshape(3, 1, 1, 1, 1, 1, 1)
+        d2 = np.sum((P - C) ** 2, axis=0)
+        out += rng.choice([-1.0, 1.0]) * amp * np.exp(-d2 / radius ** 2)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# model assembly
+# ---------------------------------------------------------------------------
+
+def make_model(box: F.Box, is_diss: bool = False, perturbation: Optional[np.ndarray] = None,
+               dlnvp=None, dlnvs=None, dlnrho=None) -> dict:
+    """Model fields in reference layout (nx+1,ny+1,nz+1,5,5,5), float32.
+
+    ``perturbation`` (relative, same for vp, vs, rho) or the individual
+    relative perturbations are applied to the 1D profile.  Fields written:
+    rhoinv=1/rho, mu=rho vs^2, lambda=rho vp^2-2 mu, A=B=C=0 (isotropic),
+    and Q (shear) when ``is_diss``; the relation of these to rho/vp/vs is
+    the one the reference uses (SOURCE/ses3d_init.f90:47-54).
+    """
+    x, y, z = gll_coords(box)
+    _, _, Z = _broadcast(box, x, y, z)
+    rho, vp, vs = prem_like(Z)
+    if perturbation is not None:
+        dlnvp = dlnvs = dlnrho = perturbation
+    if dlnvp is not None:
+        vp = vp * (1.0 + dlnvp)
+    if dlnvs is not None:
+        vs = vs * (1.0 + dlnvs)
+    if dlnrho is not None:
+        rho = rho * (1.0 + dlnrho)
+    mu = rho * vs * vs
+    lam = rho * vp * vp - 2.0 * mu
+    shape = Z.shape
+    out = {
+        "rhoinv": (1.0 / rho).astype(np.float32),
+        "mu": mu.astype(np.float32),
+        "lambda": lam.astype(np.float32),
+        "A": np.zeros(shape, np.float32),
+        "B": np.zeros(shape, np.float32),
+        "C": np.zeros(shape, np.float32),
+    }
+    if is_diss:
+        out["Q"] = q_mu_by_depth(Z).astype(np.float32)
+    return {k: np.asfortranarray(v) for k, v in out.items()}
+
+
+def ricker(nt: int, dt: float, period: float = 20.0, t0: Optional[float] = None) -> np.ndarray:
+    """Ricker wavelet of dominant period ``period`` [s], delayed by t0
+    (default 1.2 periods), sampled at t = k dt, k = 0..nt-1."""
+    f = 1.0 / period
+    if t0 is None:
+        t0 = 1.2 * period
+    t = np.arange(nt) * dt - t0
+    a = (np.pi * f * t) ** 2
+    return ((1.0 - 2.0 * a) * np.exp(-a)).astype(np.float32)
+
+
+def random_moment_tensor(rng: np.random.Generator, scale: float = 1e18) -> np.ndarray:
+    """Trace-free moment tensor (M_tt, M_pp, M_rr, M_tp, M_tr, M_pr) with
+    entries ~ N(0, scale)."""
+    m = rng.normal(0.0, scale, 6)
+    tr = (m[0] + m[1] + m[2]) / 3.0
+    m[:3] -= tr
+    return m
+
+
+def surface_receiver_grid(setup: F.Setup, n_theta: int, n_phi: int, pml: int = 2,
+                          depth: float = 0.0) -> F.Receivers:
+    """Regular grid of surface receivers inside the non-absorbing zone."""
+    nel_x, nel_y = setup.nx_global + setup.px, setup.ny_global + setup.py
+    dxd = (setup.theta_max - setup.theta_min) / nel_x
+    dyd = (setup.phi_max - setup.phi_min) / nel_y
+    t0, t1 = setup.theta_min + (pml + 0.5) * dxd, setup.theta_max - (pml + 0.5) * dxd
+    p0, p1 = setup.phi_min + (pml + 0.5) * dyd, setup.phi_max - (pml + 0.5) * dyd
+    th = np.linspace(t0, t1, n_theta)
+    ph = np.linspace(p0, p1, n_phi)
+    names, a, b = [], [], []
+    for i, t in enumerate(th):
+        for j, p in enumerate(ph):
+            names.append(f"R{i:03d}{j:03d}.XX")
+            a.append(round(float(t), 4))
+            b.append(round(float(p), 4))
+    n = len(names)
+    return F.Receivers(names, np.array(a), np.array(b), np.full(n, depth))
+
+
+def random_receivers(setup: F.Setup, n: int, rng: np.random.Generator, pml: int = 2) -> F.Receivers:
+    nel_x, nel_y = setup.nx_global + setup.px, setup.ny_global + setup.py
+    dxd = (setup.theta_max - setup.theta_min) / nel_x
+    dyd = (setup.phi_max - setup.phi_min) / nel_y
+    a = rng.uniform(setup.theta_min + (pml + 0.5) * dxd, setup.theta_max - (pml + 0.5) * dxd, n)
+    b = rng.uniform(setup.phi_min + (pml + 0.5) * dyd, setup.phi_max - (pml + 0.5) * dyd, n)
+    names = [f"S{k:05d}.YY" for k in range(n)]
+    return F.Receivers(names, np.round(a, 4), np.round(b, 4), np.zeros(n))
+
+
+# ---------------------------------------------------------------------------
+# run tree
+# ---------------------------------------------------------------------------
+
+@dataclass
+class RunCase:
+    """Everything needed to write one reference-compatible run tree."""
+    setup: F.Setup
+    events: List[F.Event]
+    receivers: List[F.Receivers]
+    stf: np.ndarray
+    model: dict
+    tau_p: Sequence[float] = (1.7308, 14.3961, 22.9973)
+    D_p: Sequence[float] = (2.5100, 2.4354, 0.0879)
+    event_names: Optional[List[str]] = None
+    adjoint: Optional[dict] = None   # event name -> (locs_deg (n,3), adsrc (n,3,nt) forward time)
+
+    def names(self) -> List[str]:
+        return self.event_names or [str(i + 1) for i in range(len(self.events))]
+
+
+def write_run_tree(root: str, case: RunCase) -> dict:
+    """Write INPUT/, MODELS/MODELS/, ADJOINT/, DATA/, MAIN/ under ``root``.
+
+    Output paths inside the event/setup files are relative to MAIN/, as the
+    reference is launched from there (MAIN/job_run_ses3d:5-7)."""
+    inp = os.path.join(root, "INPUT")
+    mod = os.path.join(root, "MODELS", "MODELS")
+    for d in (inp, mod, os.path.join(root, "MAIN"), os.path.join(root, "DATA", "LOGFILES"),
+              os.path.join(root, "DATA", "OUTPUT"), os.path.join(root, "ADJOINT")):
+        os.makedirs(d, exist_ok=True)
+    names = case.names()
+    case.setup.write(os.path.join(inp, "setup"))
+    F.write_relax(os.path.join(inp, "relax"), case.tau_p, case.D_p)
+    F.write_event_list(os.path.join(inp, "event_list"), names)
+    dt = case.events[0].dt
+    F.write_stf(os.path.join(inp, "stf"), case.stf, dt)
+    for name, ev, rec in zip(names, case.events, case.receivers):
+        ev.write(os.path.join(inp, f"event_{name}"))
+        rec.write(os.path.join(inp, f"recfile_{name}"))
+        os.makedirs(os.path.join(root, "MAIN", ev.ofd), exist_ok=True)
+    boxes = F.write_boxfile(os.path.join(mod, "boxfile"), case.setup)
+    if len(boxes) != 1:
+        raise NotImplementedError("run trees are written for one rank box (px=py=pz=1)")
+    F.write_model(mod, case.model, 0, bool(case.setup.is_diss))
+    if case.adjoint:
+        for name, (locs, adsrc) in case.adjoint.items():
+            d = os.path.join(root, "ADJOINT", name)
+            os.makedirs(d, exist_ok=True)
+            F.write_ad_srcfile(os.path.join(d, "ad_srcfile"), locs)
+            for k in range(len(locs)):
+                F.write_ad_src(os.path.join(d, f"ad_src_{k + 1}"), locs[k], adsrc[k])
+    return {"boxes": boxes, "names": names}
+
+
+def section_setup(n_theta: int, n_phi: int, n_r: int, is_diss: bool = False,
+                  adjoint_flag: int = 0, samp_ad: int = 15,
+                  ffd: str = "../DATA/OUTPUT/FWD") -> F.Setup:
+    """The BASELINE spherical section (colat 40-60, lon 10-30, r 5371-6371 km)
+    with n_theta x n_phi x n_r elements on one rank box."""
+    return F.Setup(40.0, 60.0, 10.0, 30.0, 5371000.0, 6371000.0, int(is_diss), 1,
+                   n_theta - 1, n_phi - 1, n_r - 1, 4, 1, 1, 1, adjoint_flag, samp_ad, ffd)
