@@ -1,0 +1,184 @@
+/*
+ * ses3d_hip.h -- C-ABI of the MI355X SES3D time-stepper (libses3d_hip.so).
+ *
+ * The reference has no argument-passing interface for its hot path: the
+ * Fortran driver SOURCE/ses3d_main.f90 calls argument-less subroutines that
+ * share `module variables` (SOURCE/ses3d_modules.f90:33-233).  Each entry
+ * point below replaces one of those calls or the module state it reads or
+ * writes; the citation on each names what it replaces.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative code on failure;
+ *    ses3d_hip_last_error() returns the message of the last failure on the
+ *    calling thread.
+ *  - Host arrays are caller-owned, read or written only during the call.
+ *  - Element-local arrays use the reference layout: Fortran column-major
+ *    (0:nx,0:ny,0:nz,0:4,0:4,0:4) with the element index along theta fastest.
+ *    Global arrays use (0:4(nx+1),0:4(ny+1),0:4(nz+1)).
+ *  - Angles are radians and lengths metres, i.e. the values the reference
+ *    holds after SOURCE/ses3d_input.f90 converted them (x*pi/180 in float32
+ *    with pi=3.1415926535898).
+ *  - One context holds one mesh and model (one section) and n_events events
+ *    that advance together; a context is used from one host thread.
+ *  - Results are bitwise reproducible run to run (no floating-point atomics).
+ */
+#ifndef SES3D_HIP_H
+#define SES3D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SES3D_HIP_OK 0
+#define SES3D_HIP_EINVAL (-1)
+#define SES3D_HIP_EDEVICE (-2)
+#define SES3D_HIP_ESTATE (-3)
+#define SES3D_HIP_ENOMEM (-4)
+
+typedef struct ses3d_hip_context ses3d_hip_context;
+
+/* Run parameters.  Replaces the scalars ses3d_input reads from the boxfile,
+ * setup, event and relax files (SOURCE/ses3d_input.f90:49-399) and the
+ * compile-time `pml` (SOURCE/ses3d_modules.f90:21). */
+typedef struct {
+    int32_t nx, ny, nz;        /* elements-1 along theta, phi, r (boxfile nx_max-nx_min) */
+    int32_t lpd;               /* Lagrange degree; must be 4 */
+    int32_t pml;               /* absorbing layer width in elements (reference: 2) */
+    int32_t is_diss;           /* 1: three-mechanism anelastic stress */
+    int32_t adjoint_flag;      /* 0 forward, 1 forward storing the forward field, 2 adjoint */
+    int32_t samp_ad;           /* forward-field sampling interval */
+    int32_t nt;                /* time steps */
+    float dt;                  /* time increment [s] */
+    float xmin, xmax;          /* colatitude bounds of the box [rad] */
+    float ymin, ymax;          /* longitude bounds [rad] */
+    float zmin, zmax;          /* radius bounds [m] */
+    float zmax_global;         /* radius of the surface from the setup file [m] */
+    float tau_p[3], D_p[3];    /* relaxation times and weights (INPUT/relax) */
+} ses3d_hip_config;
+
+/* Create a context on the current HIP device for n_events events sharing
+ * one mesh and model.  Replaces the static module storage of
+ * SOURCE/ses3d_modules.f90:33-233 (all sizes are run-time values here). */
+int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_context **out);
+
+/* Release all device memory of a context. */
+void ses3d_hip_destroy(ses3d_hip_context *ctx);
+
+/* Message of the last failure on this thread (empty string if none). */
+const char *ses3d_hip_last_error(void);
+
+/* Model fields, float32, reference layout; Q may be NULL when is_diss==0.
+ * Replaces the model reads of SOURCE/ses3d_input.f90:472-499. */
+int ses3d_hip_set_model(ses3d_hip_context *ctx, const float *rhoinv, const float *mu,
+                        const float *lambda, const float *A, const float *B, const float *C,
+                        const float *Q);
+
+/* Source of event ev: source_type 1/2/3 (point force theta/phi/r) or 10
+ * (moment tensor); xxs, yys in rad, zzs depth in m; mom = (Mtt,Mpp,Mrr,
+ * Mtp,Mtr,Mpr); so = nt samples of the source time function.  Replaces the
+ * source state of SOURCE/ses3d_input.f90:201-230,405-437. */
+int ses3d_hip_set_source(ses3d_hip_context *ctx, int32_t ev, int32_t source_type, float xxs,
+                         float yys, float zzs, const float *mom, const float *so);
+
+/* Receivers of event ev: rec = (nr,3) rows (colat rad, lon rad, depth m).
+ * Replaces recloc_global (SOURCE/ses3d_input.f90:531-554). */
+int ses3d_hip_set_receivers(ses3d_hip_context *ctx, int32_t ev, int32_t nr, const float *rec);
+
+/* Adjoint point sources of event ev: loc = (n,3) rows (rad, rad, m);
+ * stf = (n, nt, 3) rows as stored in ADJOINT/<ev>/ad_src_<k> (already time
+ * reversed).  Replaces SOURCE/ses3d_input.f90:584-617 and
+ * SOURCE/ses3d_init.f90:445-500. */
+int ses3d_hip_set_adjoint_sources(ses3d_hip_context *ctx, int32_t ev, int32_t n, const float *loc,
+                                  const float *stf);
+
+/* Initialise: material coefficients, GLL tables, geometry, mass matrix,
+ * receiver/source location, absorbing profiles, zero all wave fields.
+ * Replaces SOURCE/ses3d_init.f90:7-729. */
+int ses3d_hip_init(ses3d_hip_context *ctx);
+
+/* Advance all events by n_steps iterations it_begin .. it_begin+n_steps-1 of
+ * the reference time loop (SOURCE/ses3d_main.f90:184-265): the time step
+ * (SOURCE/ses3d_evolution.f90:7-373 with the sources of
+ * SOURCE/ses3d_miscellaneous.f90:11-150), seismogram recording
+ * (SOURCE/ses3d_output.f90:456-497) in forward modes, forward-field storing
+ * in HBM when adjoint_flag==1 (replaces ses3d_store, SOURCE/ses3d_output.f90:5-164)
+ * and kernel accumulation when adjoint_flag==2 (SOURCE/ses3d_grad.f90:7-89).
+ * Stream-ordered; call ses3d_hip_synchronize before reading results. */
+int ses3d_hip_run(ses3d_hip_context *ctx, int32_t it_begin, int32_t n_steps);
+
+/* Wait for all work of the context. */
+int ses3d_hip_synchronize(ses3d_hip_context *ctx);
+
+/* Number of receivers located in the box for event ev (`nr` after
+ * SOURCE/ses3d_init.f90:292-350). */
+int ses3d_hip_get_nr(ses3d_hip_context *ctx, int32_t ev, int32_t *nr);
+
+/* Index (0-based, into the rows given to set_receivers) of each located
+ * receiver, in the reference's order (station_name_local). */
+int ses3d_hip_get_receiver_index(ses3d_hip_context *ctx, int32_t ev, int32_t *idx);
+
+/* Seismograms of event ev: out = (3, nr, nt) float32 velocity, components
+ * theta, phi, r (seismogram_x/y/z, SOURCE/ses3d_modules.f90:196-198). */
+int ses3d_hip_get_seismograms(ses3d_hip_context *ctx, int32_t ev, float *out);
+
+/* Named wave field of event ev in reference layout, float32:
+ *   "vx","vy","vz"        element-local copy of the current global velocity,
+ *                         i.e. what the next step gathers.  The reference's
+ *                         snapshot of step it (SOURCE/ses3d_output.f90:304-306)
+ *                         holds the velocity gathered at the start of step it,
+ *                         so it equals this field read after step it-1.
+ *   "vx_global",...       global velocity (vx_global)
+ *   "dxux","dyuy","dzuz","dxuy","dyux","dxuz","dzux","dyuz","dzuy"
+ *   "sxx","syy","szz","sxy","syx","sxz","szx","syz","szy"   (the _pml stresses)
+ *   "Mxx","Myy","Mzz","Mxy","Mxz","Myz"  (3 mechanisms; out holds 3 fields)
+ *   "MM_global"           mass matrix
+ */
+int ses3d_hip_get_field(ses3d_hip_context *ctx, int32_t ev, const char *name, float *out);
+
+/* Frechet kernels of event ev accumulated in float64, reference layout:
+ * grad_rho, grad_cp, grad_csh, grad_csv (SOURCE/ses3d_grad.f90:77-83) and,
+ * when is_diss==1, grad_Q_mu (SOURCE/ses3d_grad_Q.f90:111-128,151-153); any
+ * pointer may be NULL.  Replaces the kernel files of SOURCE/ses3d_output.f90:318-343. */
+int ses3d_hip_get_kernels(ses3d_hip_context *ctx, int32_t ev, double *grad_rho, double *grad_cp,
+                          double *grad_csh, double *grad_csv, double *grad_Q_mu);
+
+/* Forward-field history (adjoint_flag==1 writes it, adjoint_flag==2 reads
+ * it).  A stored step holds 9 float32 fields in reference layout, in the
+ * order vx vy vz exx eyy ezz exy exz eyz -- the nine records ses3d_store
+ * writes per step (SOURCE/ses3d_main.f90:237-247).  get copies stored step
+ * `step` (1-based forward step) out; put uploads one for an adjoint run.
+ * saving_vector(it)=it if step it is stored, else 0
+ * (SOURCE/ses3d_output.f90:379-392). */
+int ses3d_hip_get_history(ses3d_hip_context *ctx, int32_t ev, int32_t step, float *out9);
+int ses3d_hip_put_history(ses3d_hip_context *ctx, int32_t ev, int32_t step, const float *in9);
+int ses3d_hip_get_saving_vector(ses3d_hip_context *ctx, int32_t *sv);
+
+/* Gradient without disk history: for an adjoint_flag==2 context, take the
+ * forward wave field from HBM checkpoints.  ses3d_hip_forward_checkpointed
+ * runs the forward problem of context fwd (adjoint_flag==1) for all nt
+ * steps keeping a restart state every k_ckpt steps in HBM instead of every
+ * sampled step; ses3d_hip_adjoint_checkpointed then runs the adjoint of
+ * context adj, replaying each forward segment from its checkpoint into an
+ * HBM buffer of sampled steps just before the adjoint consumes it.  The
+ * kernels equal those of the stored-history path bit for bit.  Replaces the
+ * disk round trip of ses3d_store/ses3d_restore (SOURCE/ses3d_output.f90:5-267,
+ * SOURCE/ses3d_grad.f90:33-45). */
+int ses3d_hip_forward_checkpointed(ses3d_hip_context *fwd, int32_t k_ckpt);
+int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *adj, ses3d_hip_context *fwd);
+
+/* Device memory held by the context [bytes]. */
+int ses3d_hip_memory_bytes(ses3d_hip_context *ctx, int64_t *bytes);
+
+/* Timing of the hot kernels over the last ses3d_hip_run call, measured with
+ * HIP events on the context's stream: total ms, and the summed ms of the
+ * element kernel and the node kernel (zero unless profiling is enabled). */
+int ses3d_hip_set_profiling(ses3d_hip_context *ctx, int32_t on);
+int ses3d_hip_get_timing(ses3d_hip_context *ctx, double *total_ms, double *element_ms,
+                         double *node_ms, int32_t *n_element_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SES3D_HIP_H */

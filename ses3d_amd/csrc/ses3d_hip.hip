@@ -1,0 +1,1292 @@
+// ses3d_hip.hip -- MI355X (gfx950) time-stepper for the SES3D spherical-section
+// spectral-element solver (lpd = 4), behind the C-ABI of include/ses3d_hip.h.
+//
+// Data layout in HBM (per section):
+//   element-local fields  f[c][e*125 + nd]   e = i + NX*(j + NY*k), nd = l + 5*(m + 5*n)
+//   global fields         v[c][blocked]      4x4x4 node blocks: node (gx,gy,gz) lives in
+//                                            block (gx>>2, gy>>2, gz>>2) at (gz&3)*16+(gy&3)*4+(gx&3)
+// Per step (SOURCE/ses3d_evolution.f90:7-373):
+//   k_receivers  seismograms from the pre-update velocity (ses3d_output.f90:475-495)
+//   k_element    gather v -> 9 velocity gradients (5-term GLL contractions through LDS)
+//                -> spherical terms -> displacement-gradient update -> absorbing-layer terms
+//                -> stress rates (elastic or 3-mechanism anelastic) -> split-stress update
+//                -> weak-form divergence (LDS) + moment source -> element force;
+//                memory-variable update, stress taper, and (adjoint) kernel accumulation fused
+//   k_forces     point forces / adjoint sources added to the element force
+//   k_node       gather the element forces of the <=8 elements sharing a node in the
+//                reference's assembly order, divide by the mass, velocity update, taper
+// All float32 arithmetic follows the reference's left-to-right order and is compiled with
+// -ffp-contract=off, so results are bit-identical to the reference's float32 build.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ses3d_hip.h"
+#include "setup.hpp"
+
+using ses3d::NN;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail(SES3D_HIP_EDEVICE, std::string(#expr ": ") + hipGetErrorString(_e));  \
+    } while (0)
+
+constexpr int EPB = 2;      // elements per block, element kernel
+constexpr int TPE = 128;    // threads per element (125 active)
+constexpr int ELEM_THREADS = EPB * TPE;
+constexpr int NODE_THREADS = 256;   // 4 node blocks of 64 per workgroup
+
+// component slots (same order for e, d, split stresses)
+enum { EXX = 0, EYY, EZZ, EXY, EYX, EXZ, EZX, EYZ, EZY };
+
+struct MeshDev {
+    int NX, NY, NZ;            // elements per axis
+    int NBX, NBY, NBZ;         // node blocks per axis (NX+1, ...)
+    long long NE, NP, NGB;     // elements, GLL points, padded global nodes
+    int KX, KY, KZ;
+    int is_diss;
+    float dt, Jac, sum_D_p;
+    float tau_p[3], Lk[3];
+    float tab[7 * 25];         // D1 D2 D3 WDL DUM1 DUM2 DUM3
+    const float *lam, *mu, *A, *B, *C, *Q, *rhoinv, *MM;
+    const float *Z, *S, *COT;
+    const unsigned short *IX, *IY, *IZ;
+    const float *LUT;
+};
+
+struct EventDev {
+    float *v;          // 3*NGB
+    float *d;          // 9*NP  dxux dyuy dzuz dxuy dyux dxuz dzux dyuz dzuy
+    float *s;          // 9*NP  sxx syy szz sxy syx sxz szx syz szy (split stresses)
+    float *M;          // 18*NP [c][k] c: xx yy zz xy xz yz, k: mechanism
+    float *f;          // 3*NP  element force
+    int src_elem;      // moment-tensor source element or -1
+    const float *src_md;   // 9*125 MOM_c*delta
+    const float *so;       // nt source time function
+    float *hist;           // sampled forward fields, 9*NP per slot
+    const int *slot;       // [it] -> slot or -1 (nt+1 entries)
+    double *grad;          // 5*NP  rho cp csh csv Q_mu
+    float sdt;             // samp_ad*dt
+};
+
+__device__ __forceinline__ long long gidx(int gx, int gy, int gz, int NBX, int NBY) {
+    return ((((long long)(gz >> 2) * NBY + (gy >> 2)) * NBX + (gx >> 2)) << 6) +
+           (((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so neighbouring elements share an L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+    int q = nb >> 3, r = nb & 7, x = bid & 7, loc = bid >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
+}
+
+// ---------------------------------------------------------------------------
+// element kernel
+// MODE 0: forward; 1: forward, store sampled fields when slot[it] >= 0;
+// 2: adjoint, accumulate kernels when slot[it] >= 0.
+// ---------------------------------------------------------------------------
+template <int DISS, int MODE>
+__global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restrict__ mesh,
+                                                          const EventDev *__restrict__ evs, int it,
+                                                          int blocks_per_event) {
+    const MeshDev &Ms = *mesh;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int ev = blk / blocks_per_event;
+    const int eb = blk - ev * blocks_per_event;
+    const EventDev &E = evs[ev];
+    const int t = threadIdx.x;
+    const int sub = t >> 7, nd = t & (TPE - 1);
+    const long long e = (long long)eb * EPB + sub;
+    const long long NE = Ms.NE, NP = Ms.NP;
+    const bool act = (nd < 125) && (e < NE);
+
+    __shared__ float tabs[7 * 25];
+    __shared__ float sv[EPB][3][TPE];
+    __shared__ float ss[EPB][9][TPE];
+    if (t < 7 * 25) tabs[t] = Ms.tab[t];
+    const float *D1 = tabs, *D2 = tabs + 25, *D3 = tabs + 50, *WDL = tabs + 75;
+    const float *DUM1 = tabs + 100, *DUM2 = tabs + 125, *DUM3 = tabs + 150;
+
+    const int l = nd % 5, m = (nd / 5) % 5, n = nd / 25;
+    const int NX = Ms.NX, NY = Ms.NY;
+    const int ei = (int)(e % NX), ej = (int)((e / NX) % NY), ek = (int)(e / ((long long)NX * NY));
+    const long long p = e * 125 + nd;
+
+    // gather v (SOURCE/ses3d_evolution.f90:27-51)
+    float vx = 0.f, vy = 0.f, vz = 0.f;
+    if (act) {
+        const long long g = gidx(4 * ei + l, 4 * ej + m, 4 * ek + n, Ms.NBX, Ms.NBY);
+        vx = E.v[g];
+        vy = E.v[Ms.NGB + g];
+        vz = E.v[2 * Ms.NGB + g];
+    }
+    sv[sub][0][nd] = vx;
+    sv[sub][1][nd] = vy;
+    sv[sub][2][nd] = vz;
+    __syncthreads();
+
+    const float dt = Ms.dt;
+    if (act) {
+        // velocity gradients by 5-term contractions (:63-87); accumulate over q in order
+        float ezz = 0.f, ezy = 0.f, ezx = 0.f, eyy = 0.f, eyx = 0.f, eyz = 0.f, exx = 0.f, exy = 0.f, exz = 0.f;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const int qz = l + 5 * (m + 5 * q), qy = l + 5 * (q + 5 * n), qx = q + 5 * (m + 5 * n);
+            const float c3 = D3[q * 5 + n], c2 = D2[q * 5 + m], c1 = D1[q * 5 + l];
+            ezz = ezz + sv[sub][2][qz] * c3;
+            ezy = ezy + sv[sub][1][qz] * c3;
+            ezx = ezx + sv[sub][0][qz] * c3;
+            eyy = eyy + sv[sub][1][qy] * c2;
+            eyx = eyx + sv[sub][0][qy] * c2;
+            eyz = eyz + sv[sub][2][qy] * c2;
+            exx = exx + sv[sub][0][qx] * c1;
+            exy = exy + sv[sub][1][qx] * c1;
+            exz = exz + sv[sub][2][qx] * c1;
+        }
+        // spherical completion (:93-103)
+        const float r = Ms.Z[ek * 5 + n], st = Ms.S[ei * 5 + l], ct = Ms.COT[ei * 5 + l];
+        const float rst = r * st;
+        ezz = -ezz;
+        eyy = eyy / rst + (vz + vx * ct) / r;
+        exx = (vz + exx) / r;
+        ezx = -ezx;
+        exz = (exz - vx) / r;
+        ezy = -ezy;
+        eyz = -vy / r + eyz / rst;
+        exy = exy / r;
+        eyx = -((vy * ct) / r) + eyx / rst;
+        // displacement gradients (:109-117)
+        float *d = E.d;
+        float dxux = d[EXX * NP + p], dyuy = d[EYY * NP + p], dzuz = d[EZZ * NP + p];
+        float dxuy = d[EXY * NP + p], dyux = d[EYX * NP + p], dxuz = d[EXZ * NP + p];
+        float dzux = d[EZX * NP + p], dyuz = d[EYZ * NP + p], dzuy = d[EZY * NP + p];
+        dzuz = dzuz + dt * ezz;
+        dyuy = dyuy + dt * eyy;
+        dxux = dxux + dt * exx;
+        dzux = dzux + dt * ezx;
+        dxuz = dxuz + dt * exz;
+        dzuy = dzuy + dt * ezy;
+        dyuz = dyuz + dt * eyz;
+        dxuy = dxuy + dt * exy;
+        dyux = dyux + dt * eyx;
+        d[EXX * NP + p] = dxux; d[EYY * NP + p] = dyuy; d[EZZ * NP + p] = dzuz;
+        d[EXY * NP + p] = dxuy; d[EYX * NP + p] = dyux; d[EXZ * NP + p] = dxuz;
+        d[EZX * NP + p] = dzux; d[EYZ * NP + p] = dyuz; d[EZY * NP + p] = dzuy;
+        // absorbing-layer terms (:123-134); exact no-op outside the layers
+        float px = 0.f, py = 0.f, pz = 0.f, taper = 1.f;
+        const unsigned ix = Ms.IX[ei * 5 + l], iy = Ms.IY[ej * 5 + m], iz = Ms.IZ[ek * 5 + n];
+        if (ix | iy | iz) {
+            const float *lut = Ms.LUT + (((size_t)iz * Ms.KY + iy) * Ms.KX + ix) * 5;
+            px = lut[0]; py = lut[1]; pz = lut[2]; taper = lut[4];
+            exx = exx + (pz + py) * dxux;
+            eyy = eyy + (px + pz) * dyuy;
+            ezz = ezz + (px + py) * dzuz;
+            exy = exy + (py + pz) * dxuy;
+            eyx = eyx + (px + pz) * dyux;
+            exz = exz + (py + pz) * dxuz;
+            ezx = ezx + (px + py) * dzux;
+            eyz = eyz + (px + pz) * dyuz;
+            ezy = ezy + (px + py) * dzuy;
+        }
+        exy = (exy + eyx) / 2.0f;
+        exz = (exz + ezx) / 2.0f;
+        eyz = (eyz + ezy) / 2.0f;
+        // stress rates (:146-202)
+        const float mu = Ms.mu[p], A = Ms.A[p], B = Ms.B[p], C = Ms.C[p];
+        float sxx, syy, szz, sxy, sxz, syz;
+        float Mv[18];
+        if (DISS) {
+            const float QQ = Ms.Q[p];
+            const float tau = 1.0f / QQ;
+            const float kappa = Ms.lam[p] + (2.0f * mu) / 3.0f;
+            const float mut = (1.0f + Ms.sum_D_p * tau) * mu;
+#pragma unroll
+            for (int c = 0; c < 18; ++c) Mv[c] = E.M[c * NP + p];
+            float Mdx = ((0.f + Mv[0]) + Mv[1]) + Mv[2];
+            float Mdy = ((0.f + Mv[3]) + Mv[4]) + Mv[5];
+            float Mdz = ((0.f + Mv[6]) + Mv[7]) + Mv[8];
+            const float Md = (Mdx + Mdy) + Mdz;
+            const float mt2 = (2.0f * mu) * tau;
+            const float L = (kappa - (2.0f * mut) / 3.0f) * ((exx + eyy) + ezz) - (mt2 * Md) / 3.0f;
+            sxx = (((L + (2.0f * mut) * exx) + mt2 * Mdx) + C * ezz) + A * (eyy + exx);
+            syy = (((L + (2.0f * mut) * eyy) + mt2 * Mdy) + C * ezz) + A * (eyy + exx);
+            szz = ((L + (2.0f * mut) * ezz) + mt2 * Mdz) + C * (eyy + exx);
+            Mdx = ((0.f + Mv[9]) + Mv[10]) + Mv[11];
+            Mdy = ((0.f + Mv[12]) + Mv[13]) + Mv[14];
+            Mdz = ((0.f + Mv[15]) + Mv[16]) + Mv[17];
+            sxy = (2.0f * mut) * exy + mt2 * Mdx;
+            sxz = ((2.0f * mut) * exz + mt2 * Mdy) + (2.0f * B) * exz;
+            syz = ((2.0f * mut) * eyz + mt2 * Mdz) + (2.0f * B) * eyz;
+            // memory variables (:334-360), from the post-absorbing, symmetrised rates
+            const float ev6[6] = {exx, eyy, ezz, exy, exz, eyz};
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float Mo = Mv[c * 3 + k];
+                    const float Mn = (Mo - (dt * Mo) / Ms.tau_p[k]) - Ms.Lk[k] * ev6[c];
+                    Mv[c * 3 + k] = Mn;
+                    E.M[(c * 3 + k) * NP + p] = Mn;
+                }
+        } else {
+            const float L = Ms.lam[p] * ((exx + eyy) + ezz);
+            sxx = ((L + (2.0f * mu) * exx) + C * ezz) + A * (eyy + exx);
+            syy = ((L + (2.0f * mu) * eyy) + C * ezz) + A * (eyy + exx);
+            szz = (L + (2.0f * mu) * ezz) + C * (eyy + exx);
+            sxy = (2.0f * mu) * exy;
+            sxz = (2.0f * mu) * exz + (2.0f * B) * exz;
+            syz = (2.0f * mu) * eyz + (2.0f * B) * eyz;
+        }
+        // split-stress integration (:208-219)
+        float *s = E.s;
+        float s9[9];
+        s9[EXX] = s[EXX * NP + p]; s9[EYY] = s[EYY * NP + p]; s9[EZZ] = s[EZZ * NP + p];
+        s9[EXY] = s[EXY * NP + p]; s9[EYX] = s[EYX * NP + p]; s9[EXZ] = s[EXZ * NP + p];
+        s9[EZX] = s[EZX * NP + p]; s9[EYZ] = s[EYZ * NP + p]; s9[EZY] = s[EZY * NP + p];
+        s9[EXX] = s9[EXX] + dt * (sxx - px * s9[EXX]);
+        s9[EYY] = s9[EYY] + dt * (syy - py * s9[EYY]);
+        s9[EZZ] = s9[EZZ] + dt * (szz - pz * s9[EZZ]);
+        s9[EXY] = s9[EXY] + dt * (sxy - px * s9[EXY]);
+        s9[EYX] = s9[EYX] + dt * (sxy - py * s9[EYX]);
+        s9[EXZ] = s9[EXZ] + dt * (sxz - px * s9[EXZ]);
+        s9[EZX] = s9[EZX] + dt * (sxz - pz * s9[EZX]);
+        s9[EYZ] = s9[EYZ] + dt * (syz - py * s9[EYZ]);
+        s9[EZY] = s9[EZY] + dt * (syz - pz * s9[EZY]);
+        // moment-tensor stress glut (misc.f90:131-142, added at :251-264)
+        if (e == E.src_elem) {
+            const float so = E.so[it - 1];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) ss[sub][c][nd] = s9[c] + (0.0f - E.src_md[c * 125 + nd] * so);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) ss[sub][c][nd] = s9[c];
+        }
+        // stress taper (:368-371); the divergence uses the untapered stresses
+#pragma unroll
+        for (int c = 0; c < 9; ++c) s[c * NP + p] = s9[c] * taper;
+
+        if (MODE == 1) {
+            // forward-field store (ses3d_main.f90:237-247)
+            const int sl = E.slot[it];
+            if (sl >= 0) {
+                float *h = E.hist + (long long)sl * 9 * NP;
+                h[0 * NP + p] = vx; h[1 * NP + p] = vy; h[2 * NP + p] = vz;
+                h[3 * NP + p] = dxux; h[4 * NP + p] = dyuy; h[5 * NP + p] = dzuz;
+                h[6 * NP + p] = (dxuy + dyux) / 2.0f;
+                h[7 * NP + p] = (dxuz + dzux) / 2.0f;
+                h[8 * NP + p] = (dyuz + dzuy) / 2.0f;
+            }
+        }
+        if (MODE == 2) {
+            // kernels (SOURCE/ses3d_grad.f90:53-83, ses3d_grad_Q.f90:115-126); float32
+            // increments in the reference's order, accumulated in float64
+            const int sl = E.slot[it];
+            if (sl >= 0) {
+                const float *h = E.hist + (long long)sl * 9 * NP;
+                const float vxf = h[p], vyf = h[NP + p], vzf = h[2 * NP + p];
+                const float exxf = h[3 * NP + p], eyyf = h[4 * NP + p], ezzf = h[5 * NP + p];
+                const float exyf = h[6 * NP + p], exzf = h[7 * NP + p], eyzf = h[8 * NP + p];
+                const float exya = 0.5f * (dxuy + dyux), exza = 0.5f * (dxuz + dzux), eyza = 0.5f * (dyuz + dzuy);
+                const float tra = (dxux + dyuy) + dzuz, trf = (exxf + eyyf) + ezzf;
+                const float ede = ((((dxux * exxf + dyuy * eyyf) + dzuz * ezzf) + (2.0f * exya) * exyf) +
+                                   (2.0f * exza) * exzf) + (2.0f * eyza) * eyzf;
+                const float rhoinv = Ms.rhoinv[p], lam = Ms.lam[p];
+                const float rho = rhoinv > 0.f ? 1.0f / rhoinv : 0.f;
+                const float cs = sqrtf(mu * rhoinv);
+                const float cp = sqrtf((lam + 2.0f * mu) * rhoinv);
+                const float sdt = E.sdt, Jac = Ms.Jac;
+                double *gr = E.grad;
+                const float g_cp = (sdt * (((2.0f * rho) * cp) * trf * tra)) / Jac;
+                const float g_rho1 = (sdt * ((vxf * vx + vyf * vy) + vzf * vz)) / Jac;
+                const float g_rho2 = (sdt * (((cp * cp - (2.0f * cs) * cs) * trf) * tra + ((2.0f * cs) * cs) * ede)) / Jac;
+                const float g_csh = (sdt * (((4.0f * rho) * cs) *
+                                            (((ede - tra * trf) - (2.0f * exzf) * exza) - (2.0f * eyzf) * eyza))) / Jac;
+                const float g_csv = (sdt * (((8.0f * rho) * cs) * (exzf * exza + eyzf * eyza))) / Jac;
+                gr[p] = (gr[p] + (double)g_rho1) + (double)g_rho2;
+                gr[NP + p] = gr[NP + p] + (double)g_cp;
+                gr[2 * NP + p] = gr[2 * NP + p] + (double)g_csh;
+                gr[3 * NP + p] = gr[3 * NP + p] + (double)g_csv;
+                if (DISS) {
+                    double gq = gr[4 * NP + p];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float Mxx = Mv[0 * 3 + k], Myy = Mv[1 * 3 + k], Mzz = Mv[2 * 3 + k];
+                        const float Mxy = Mv[3 * 3 + k], Mxz = Mv[4 * 3 + k], Myz = Mv[5 * 3 + k];
+                        const float md = (((((Mxx * exxf + Myy * eyyf) + Mzz * ezzf) + Mxy * exyf) + Mxz * exzf) +
+                                          Myz * eyzf) - (trf * ((Mxx + Myy) + Mzz)) / 3.0f;
+                        gq = gq + (double)(((sdt * md) * Ms.tau_p[k]) / Jac);
+                    }
+                    gr[4 * NP + p] = gq;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    if (act) {
+        // weak-form stress divergence (:237-270)
+        const float zc = Ms.Z[ek * 5 + n], sa = Ms.S[ei * 5 + l];
+        float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const float wx = ((WDL[l * 5 + q] * zc) * Ms.S[ei * 5 + q]) * DUM1[m * 5 + n];
+            const float wy = (WDL[m * 5 + q] * zc) * DUM2[l * 5 + n];
+            const float zq = Ms.Z[ek * 5 + q];
+            const float wz = (((WDL[n * 5 + q] * zq) * zq) * sa) * DUM3[l * 5 + m];
+            const int pz_ = l + 5 * (m + 5 * q), py_ = l + 5 * (q + 5 * n), px_ = q + 5 * (m + 5 * n);
+            ax = ((ax + ss[sub][EZX][pz_] * wz) + ss[sub][EYX][py_] * wy) + ss[sub][EXX][px_] * wx;
+            ay = ((ay + ss[sub][EZY][pz_] * wz) + ss[sub][EYY][py_] * wy) + ss[sub][EXY][px_] * wx;
+            az = ((az + ss[sub][EZZ][pz_] * wz) + ss[sub][EYZ][py_] * wy) + ss[sub][EXZ][px_] * wx;
+        }
+        E.f[p] = ax;
+        E.f[NP + p] = ay;
+        E.f[2 * NP + p] = az;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// point forces (forward source_type 1/2/3, adjoint sources): misc.f90:29-92.
+// One block per (event, source element); each thread one node; the element's
+// sources are applied in their reference order.
+// ---------------------------------------------------------------------------
+struct ForceGroup {
+    int ev;
+    long long elem;
+    int first, count;   // into the force list
+};
+struct ForceDev {
+    int comp_mask;        // bit c: subtract from component c
+    const float *w;       // 125 weights
+    const float *stf;     // 3*nt: component c at stf[c*nt + it-1]
+};
+
+__global__ void k_forces(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs,
+                         const ForceGroup *__restrict__ groups, const ForceDev *__restrict__ forces,
+                         int nt, int it) {
+    const ForceGroup G = groups[blockIdx.x];
+    const int nd = threadIdx.x;
+    if (nd >= 125) return;
+    const long long NP = mesh->NP;
+    float *f = evs[G.ev].f;
+    const long long p = G.elem * 125 + nd;
+    float fx = f[p], fy = f[NP + p], fz = f[2 * NP + p];
+    for (int s = G.first; s < G.first + G.count; ++s) {
+        const ForceDev F = forces[s];
+        const float w = F.w[nd];
+        if (F.comp_mask & 1) fx = fx - w * F.stf[0 * nt + it - 1];
+        if (F.comp_mask & 2) fy = fy - w * F.stf[1 * nt + it - 1];
+        if (F.comp_mask & 4) fz = fz - w * F.stf[2 * nt + it - 1];
+    }
+    f[p] = fx; f[NP + p] = fy; f[2 * NP + p] = fz;
+}
+
+// ---------------------------------------------------------------------------
+// node kernel: assembly in the reference's order (sharing elements in
+// lexicographic (i,j,k) order, :286-310), /MM (:318-320), velocity update
+// (:326-328), taper (:366).  One thread per global node; a wavefront owns one
+// 4x4x4 node block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict__ mesh,
+                                                       const EventDev *__restrict__ evs,
+                                                       int nodeblocks_per_event, int n_events) {
+    const MeshDev &Ms = *mesh;
+    const long long gt = (long long)xcd_remap(blockIdx.x, gridDim.x) * NODE_THREADS + threadIdx.x;
+    const long long nbt = gt >> 6;
+    const int ev = (int)(nbt / nodeblocks_per_event);
+    if (ev >= n_events) return;
+    const long long nb = nbt - (long long)ev * nodeblocks_per_event;
+    const int loc = (int)(gt & 63);
+    const int NX = Ms.NX, NY = Ms.NY, NZ = Ms.NZ, NBX = Ms.NBX, NBY = Ms.NBY;
+    const int bx = (int)(nb % NBX), by = (int)((nb / NBX) % NBY), bz = (int)(nb / ((long long)NBX * NBY));
+    const int a = loc & 3, b = (loc >> 2) & 3, c = loc >> 4;
+    if (bz >= Ms.NBZ) return;
+    if ((bx == NX && a) || (by == NY && b) || (bz == NZ && c)) return;
+    const EventDev &E = evs[ev];
+    const long long NP = Ms.NP;
+    // sharers per axis, ascending element index
+    int xe[2], xl[2], nxs = 0, ye[2], yl[2], nys = 0, ze[2], zl[2], nzs = 0;
+    if (a) { xe[0] = bx; xl[0] = a; nxs = 1; }
+    else {
+        if (bx > 0) { xe[nxs] = bx - 1; xl[nxs] = 4; ++nxs; }
+        if (bx < NX) { xe[nxs] = bx; xl[nxs] = 0; ++nxs; }
+    }
+    if (b) { ye[0] = by; yl[0] = b; nys = 1; }
+    else {
+        if (by > 0) { ye[nys] = by - 1; yl[nys] = 4; ++nys; }
+        if (by < NY) { ye[nys] = by; yl[nys] = 0; ++nys; }
+    }
+    if (c) { ze[0] = bz; zl[0] = c; nzs = 1; }
+    else {
+        if (bz > 0) { ze[nzs] = bz - 1; zl[nzs] = 4; ++nzs; }
+        if (bz < NZ) { ze[nzs] = bz; zl[nzs] = 0; ++nzs; }
+    }
+    float Sx = 0.f, Sy = 0.f, Sz = 0.f;
+    for (int ii = 0; ii < nxs; ++ii)
+        for (int jj = 0; jj < nys; ++jj)
+            for (int kk = 0; kk < nzs; ++kk) {
+                const long long q = ((long long)xe[ii] + (long long)NX * (ye[jj] + (long long)NY * ze[kk])) * 125 +
+                                    (xl[ii] + 5 * (yl[jj] + 5 * zl[kk]));
+                Sx = Sx + E.f[q];
+                Sy = Sy + E.f[NP + q];
+                Sz = Sz + E.f[2 * NP + q];
+            }
+    // absorbing profile and taper of the global node: the reference copies the
+    // element-local value of the last element in loop order (init.f90:690-709)
+    float pg = 0.f, tg = 1.f;
+    {
+        const int exl = bx < NX ? bx : NX - 1, eyl = by < NY ? by : NY - 1, ezl = bz < NZ ? bz : NZ - 1;
+        const int lxl = 4 * bx + a - 4 * exl, lyl = 4 * by + b - 4 * eyl, lzl = 4 * bz + c - 4 * ezl;
+        const unsigned ix = Ms.IX[exl * 5 + lxl], iy = Ms.IY[eyl * 5 + lyl], iz = Ms.IZ[ezl * 5 + lzl];
+        if (ix | iy | iz) {
+            const float *lut = Ms.LUT + (((size_t)iz * Ms.KY + iy) * Ms.KX + ix) * 5;
+            pg = lut[3]; tg = lut[4];
+        }
+    }
+    const long long g = (nb << 6) + loc;
+    const float mm = Ms.MM[g];
+    const float dt = Ms.dt;
+    float *v = E.v;
+    const float ax = Sx / mm, ay = Sy / mm, az = Sz / mm;
+    float vx = v[g], vy = v[Ms.NGB + g], vz = v[2 * Ms.NGB + g];
+    vx = vx + dt * (ax - pg * vx);
+    vy = vy + dt * (ay - pg * vy);
+    vz = vz + dt * (az - pg * vz);
+    v[g] = tg * vx;
+    v[Ms.NGB + g] = tg * vy;
+    v[2 * Ms.NGB + g] = tg * vz;
+}
+
+// mass matrix (SOURCE/ses3d_init.f90:254-277): the same ordered gather as k_node
+__global__ __launch_bounds__(NODE_THREADS) void k_mass(MeshDev *mesh, float *MM, const float *w5) {
+    const MeshDev &Ms = *mesh;
+    const long long gt = (long long)blockIdx.x * NODE_THREADS + threadIdx.x;
+    const long long nb = gt >> 6;
+    const int loc = (int)(gt & 63);
+    const int NX = Ms.NX, NY = Ms.NY, NZ = Ms.NZ, NBX = Ms.NBX, NBY = Ms.NBY;
+    const int bx = (int)(nb % NBX), by = (int)((nb / NBX) % NBY), bz = (int)(nb / ((long long)NBX * NBY));
+    const int a = loc & 3, b = (loc >> 2) & 3, c = loc >> 4;
+    if (bz >= Ms.NBZ) return;
+    if ((bx == NX && a) || (by == NY && b) || (bz == NZ && c)) { MM[gt] = 1.0f; return; }
+    int xe[2], xl[2], nxs = 0, ye[2], yl[2], nys = 0, ze[2], zl[2], nzs = 0;
+    if (a) { xe[0] = bx; xl[0] = a; nxs = 1; }
+    else {
+        if (bx > 0) { xe[nxs] = bx - 1; xl[nxs] = 4; ++nxs; }
+        if (bx < NX) { xe[nxs] = bx; xl[nxs] = 0; ++nxs; }
+    }
+    if (b) { ye[0] = by; yl[0] = b; nys = 1; }
+    else {
+        if (by > 0) { ye[nys] = by - 1; yl[nys] = 4; ++nys; }
+        if (by < NY) { ye[nys] = by; yl[nys] = 0; ++nys; }
+    }
+    if (c) { ze[0] = bz; zl[0] = c; nzs = 1; }
+    else {
+        if (bz > 0) { ze[nzs] = bz - 1; zl[nzs] = 4; ++nzs; }
+        if (bz < NZ) { ze[nzs] = bz; zl[nzs] = 0; ++nzs; }
+    }
+    float mm = 0.f;
+    for (int ii = 0; ii < nxs; ++ii)
+        for (int jj = 0; jj < nys; ++jj)
+            for (int kk = 0; kk < nzs; ++kk) {
+                const int i = xe[ii], j = ye[jj], k = ze[kk], L = xl[ii], Mi = yl[jj], N = zl[kk];
+                const long long q = ((long long)i + (long long)NX * (j + (long long)NY * k)) * 125 + (L + 5 * (Mi + 5 * N));
+                const float zk = Ms.Z[k * 5 + N];
+                const float tterm = (((((((w5[L] * w5[Mi]) * w5[N]) * Ms.Jac) * zk) * zk) * Ms.S[i * 5 + L])) / Ms.rhoinv[q];
+                mm = mm - tterm;
+            }
+    MM[gt] = mm;
+}
+
+// seismograms (SOURCE/ses3d_output.f90:477-491): one thread per (event, receiver,
+// component); weights precomputed per receiver; summed in the reference's i,j,k order.
+struct RecDev {
+    int ev;
+    int ex, ey, ez;
+    const float *w;     // 125 weights, index l + 5m + 25n
+    float *out;         // 3 x nt for this receiver (component-major)
+};
+
+__global__ void k_receivers(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs,
+                            const RecDev *__restrict__ recs, int nrec, int nt, int it) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = t / 3, comp = t - 3 * r;
+    if (r >= nrec) return;
+    const RecDev R = recs[r];
+    const MeshDev &Ms = *mesh;
+    const float *v = evs[R.ev].v + comp * Ms.NGB;
+    float acc = 0.f;
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j)
+            for (int k = 0; k < 5; ++k) {
+                const long long g = gidx(4 * R.ex + i, 4 * R.ey + j, 4 * R.ez + k, Ms.NBX, Ms.NBY);
+                acc = acc + v[g] * R.w[i + 5 * (j + 5 * k)];
+            }
+    R.out[comp * nt + it - 1] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct EventHost {
+    int source_type = 0;
+    float xxs = 0, yys = 0, zzs = 0, mom[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<float> so;
+    bool have_source = false;
+    std::vector<float> rec;   // 3*nr
+    std::vector<float> ad_loc, ad_stf;   // 3*n, n*nt*3
+    // located
+    ses3d::SourceLoc sloc;
+    std::vector<ses3d::PointLoc> recs, ads;
+    // device
+    float *v = nullptr, *d = nullptr, *s = nullptr, *M = nullptr, *f = nullptr;
+    float *src_md = nullptr, *so_d = nullptr;
+    float *seis = nullptr;     // nr x 3 x nt
+    float *rec_w = nullptr;    // nr x 125
+    float *force_w = nullptr, *force_stf = nullptr;
+    double *grad = nullptr;
+    float *hist = nullptr;
+    int n_slots = 0;
+    std::vector<int> slot_h;   // per forward step: history slot (adjoint_flag 1/2 store)
+    int *slot_d = nullptr;
+    // checkpoints (forward_checkpointed)
+    std::vector<float *> ckpt;
+};
+
+struct ses3d_hip_context {
+    ses3d_hip_config cfg;
+    int n_events = 0;
+    int NX = 0, NY = 0, NZ = 0, NBX = 0, NBY = 0, NBZ = 0;
+    long long NE = 0, NP = 0, NGB = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    ses3d::MeshSetup ms;
+    bool model_set = false, initialised = false;
+    // device mesh data
+    float *lam = nullptr, *mu = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *Q = nullptr,
+          *rhoinv = nullptr, *MM = nullptr;
+    float *Z = nullptr, *S = nullptr, *COT = nullptr, *LUT = nullptr, *w5 = nullptr;
+    unsigned short *IX = nullptr, *IY = nullptr, *IZ = nullptr;
+    MeshDev mesh_h{};
+    MeshDev *mesh_d = nullptr;
+    std::vector<EventHost> evh;
+    EventDev *ev_d = nullptr;
+    // receivers / forces (all events)
+    RecDev *recs_d = nullptr;
+    int nrec_total = 0;
+    ForceGroup *groups_d = nullptr;
+    ForceDev *forces_d = nullptr;
+    int n_groups = 0;
+    std::vector<int> saving_vector;   // 1..nt
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    // timing
+    bool profiling = false;
+    double t_total = 0, t_elem = 0, t_node = 0;
+    int n_elem = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> tev;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(ses3d_hip_context *c, T **p, size_t n) {
+    void *q = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return fail(SES3D_HIP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    c->allocs.push_back(q);
+    c->bytes += (int64_t)bytes;
+    *p = (T *)q;
+    return 0;
+}
+
+#define TRY(expr)                       \
+    do {                                \
+        int _rc = (expr);               \
+        if (_rc) return _rc;            \
+    } while (0)
+
+// reference layout (element fastest, node slowest) <-> element-major layout
+void ref_to_em(const ses3d_hip_context *c, const float *src, float *dst) {
+    const long long NE = c->NE;
+    for (int nd = 0; nd < 125; ++nd)
+        for (long long e = 0; e < NE; ++e) dst[e * 125 + nd] = src[e + NE * nd];
+}
+template <typename T>
+void em_to_ref(const ses3d_hip_context *c, const T *src, T *dst) {
+    const long long NE = c->NE;
+    for (int nd = 0; nd < 125; ++nd)
+        for (long long e = 0; e < NE; ++e) dst[e + NE * nd] = src[e * 125 + nd];
+}
+
+long long host_gidx(const ses3d_hip_context *c, int gx, int gy, int gz) {
+    return ((((long long)(gz >> 2) * c->NBY + (gy >> 2)) * c->NBX + (gx >> 2)) << 6) +
+           (((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
+}
+
+int check_ev(ses3d_hip_context *c, int ev) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (ev < 0 || ev >= c->n_events) return fail(SES3D_HIP_EINVAL, "event index out of range");
+    return 0;
+}
+
+int upload(ses3d_hip_context *c, void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int download(ses3d_hip_context *c, void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int zero(ses3d_hip_context *c, void *p, size_t bytes) {
+    // a fill kernel would also do; the context's stream is never captured here
+    HIP_TRY(hipMemsetAsync(p, 0, bytes, c->stream));
+    return 0;
+}
+
+int elem_blocks(const ses3d_hip_context *c) { return (int)((c->NE + EPB - 1) / EPB); }
+long long node_blocks(const ses3d_hip_context *c) { return (long long)c->NBX * c->NBY * c->NBZ; }
+
+int launch_element(ses3d_hip_context *c, int it, int mode) {
+    const int bpe = elem_blocks(c);
+    dim3 grid((unsigned)(bpe * c->n_events)), block(ELEM_THREADS);
+    const bool diss = c->cfg.is_diss != 0;
+#define LAUNCH_E(D, MO) hipLaunchKernelGGL((k_element<D, MO>), grid, block, 0, c->stream, c->mesh_d, c->ev_d, it, bpe)
+    if (diss) {
+        if (mode == 0) LAUNCH_E(1, 0); else if (mode == 1) LAUNCH_E(1, 1); else LAUNCH_E(1, 2);
+    } else {
+        if (mode == 0) LAUNCH_E(0, 0); else if (mode == 1) LAUNCH_E(0, 1); else LAUNCH_E(0, 2);
+    }
+#undef LAUNCH_E
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_node(ses3d_hip_context *c) {
+    const long long nb = node_blocks(c);
+    const long long threads = nb * 64 * c->n_events;
+    dim3 grid((unsigned)((threads + NODE_THREADS - 1) / NODE_THREADS)), block(NODE_THREADS);
+    hipLaunchKernelGGL(k_node, grid, block, 0, c->stream, c->mesh_d, c->ev_d, (int)nb, c->n_events);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// One iteration of the reference time loop for all events.
+int step(ses3d_hip_context *c, int it, int mode, bool record) {
+    const int nt = c->cfg.nt;
+    if (record && c->nrec_total > 0) {
+        const int threads = 3 * c->nrec_total;
+        hipLaunchKernelGGL(k_receivers, dim3((threads + 127) / 128), dim3(128), 0, c->stream, c->mesh_d,
+                           c->ev_d, c->recs_d, c->nrec_total, nt, it);
+        HIP_TRY(hipGetLastError());
+    }
+    if (c->profiling) HIP_TRY(hipEventRecord(c->tev[0], c->stream));
+    TRY(launch_element(c, it, mode));
+    if (c->profiling) HIP_TRY(hipEventRecord(c->tev[1], c->stream));
+    if (c->n_groups > 0) {
+        hipLaunchKernelGGL(k_forces, dim3(c->n_groups), dim3(128), 0, c->stream, c->mesh_d, c->ev_d,
+                           c->groups_d, c->forces_d, nt, it);
+        HIP_TRY(hipGetLastError());
+    }
+    if (c->profiling) HIP_TRY(hipEventRecord(c->tev[2], c->stream));
+    TRY(launch_node(c));
+    if (c->profiling) {
+        HIP_TRY(hipEventRecord(c->tev[3], c->stream));
+        HIP_TRY(hipEventSynchronize(c->tev[3]));
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, c->tev[0], c->tev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, c->tev[2], c->tev[3]));
+        c->t_elem += a;
+        c->t_node += b;
+        c->n_elem += 1;
+    }
+    return 0;
+}
+
+int build_slots(ses3d_hip_context *c, EventHost &E, bool allocate_history) {
+    const int nt = c->cfg.nt, sa = c->cfg.samp_ad > 0 ? c->cfg.samp_ad : 1;
+    E.slot_h.assign(nt + 1, -1);
+    int n = 0;
+    for (int it = 1; it <= nt; ++it)
+        if ((it % sa) == 0 || it == 1 || it == nt) E.slot_h[it] = n++;
+    E.n_slots = n;
+    if (!E.slot_d) TRY(dalloc(c, &E.slot_d, nt + 1));
+    if (allocate_history && !E.hist) TRY(dalloc(c, &E.hist, (size_t)n * 9 * c->NP));
+    return 0;
+}
+
+int refresh_events(ses3d_hip_context *c) {
+    std::vector<EventDev> h(c->n_events);
+    for (int ev = 0; ev < c->n_events; ++ev) {
+        EventHost &E = c->evh[ev];
+        EventDev &D = h[ev];
+        D.v = E.v; D.d = E.d; D.s = E.s; D.M = E.M; D.f = E.f;
+        D.src_elem = -1;
+        if (E.have_source && E.source_type == 10 && E.sloc.is_source &&
+            (c->cfg.adjoint_flag == 0 || c->cfg.adjoint_flag == 1))
+            D.src_elem = (int)(E.sloc.isx + (long long)c->NX * (E.sloc.isy + (long long)c->NY * E.sloc.isz));
+        D.src_md = E.src_md;
+        D.so = E.so_d;
+        D.hist = E.hist;
+        D.slot = E.slot_d;
+        D.grad = E.grad;
+        D.sdt = (float)c->cfg.samp_ad * c->cfg.dt;
+    }
+    return upload(c, c->ev_d, h.data(), sizeof(EventDev) * c->n_events);
+}
+
+}  // namespace
+
+// ============================================================================
+// C-ABI
+// ============================================================================
+extern "C" {
+
+const char *ses3d_hip_last_error(void) { return g_err.c_str(); }
+
+int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_context **out) {
+    if (!cfg || !out) return fail(SES3D_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (cfg->lpd != 4) return fail(SES3D_HIP_EINVAL, "only lpd=4 is supported");
+    if (cfg->nx < 0 || cfg->ny < 0 || cfg->nz < 0) return fail(SES3D_HIP_EINVAL, "negative element count");
+    if (cfg->nt < 1) return fail(SES3D_HIP_EINVAL, "nt must be >= 1");
+    if (n_events < 1) return fail(SES3D_HIP_EINVAL, "n_events must be >= 1");
+    if (cfg->pml < 0 || cfg->pml > cfg->nx + 1 || cfg->pml > cfg->ny + 1 || cfg->pml > cfg->nz + 1)
+        return fail(SES3D_HIP_EINVAL, "pml width exceeds the mesh");
+    if (cfg->adjoint_flag < 0 || cfg->adjoint_flag > 2) return fail(SES3D_HIP_EINVAL, "adjoint_flag must be 0, 1 or 2");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(SES3D_HIP_EDEVICE, "no HIP device available");
+    auto *c = new ses3d_hip_context();
+    c->cfg = *cfg;
+    c->n_events = n_events;
+    c->NX = cfg->nx + 1; c->NY = cfg->ny + 1; c->NZ = cfg->nz + 1;
+    c->NBX = c->NX + 1; c->NBY = c->NY + 1; c->NBZ = c->NZ + 1;
+    c->NE = (long long)c->NX * c->NY * c->NZ;
+    c->NP = c->NE * 125;
+    c->NGB = (long long)c->NBX * c->NBY * c->NBZ * 64;
+    hipGetDevice(&c->device);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(SES3D_HIP_EDEVICE, "hipStreamCreate failed");
+    }
+    c->tev.resize(4);
+    for (auto &e : c->tev) hipEventCreate(&e);
+    c->evh.resize(n_events);
+    c->ms.build(cfg->nx, cfg->ny, cfg->nz, cfg->pml, cfg->xmin, cfg->xmax, cfg->ymin, cfg->ymax, cfg->zmin,
+                cfg->zmax, cfg->zmax_global, cfg->dt);
+    int rc = 0;
+    const long long NP = c->NP, NGB = c->NGB;
+    rc = rc ? rc : dalloc(c, &c->lam, NP);
+    rc = rc ? rc : dalloc(c, &c->mu, NP);
+    rc = rc ? rc : dalloc(c, &c->A, NP);
+    rc = rc ? rc : dalloc(c, &c->B, NP);
+    rc = rc ? rc : dalloc(c, &c->C, NP);
+    rc = rc ? rc : dalloc(c, &c->rhoinv, NP);
+    if (cfg->is_diss) rc = rc ? rc : dalloc(c, &c->Q, NP);
+    rc = rc ? rc : dalloc(c, &c->MM, NGB);
+    rc = rc ? rc : dalloc(c, &c->mesh_d, 1);
+    rc = rc ? rc : dalloc(c, &c->ev_d, n_events);
+    for (int ev = 0; ev < n_events && !rc; ++ev) {
+        EventHost &E = c->evh[ev];
+        rc = rc ? rc : dalloc(c, &E.v, 3 * NGB);
+        rc = rc ? rc : dalloc(c, &E.d, 9 * NP);
+        rc = rc ? rc : dalloc(c, &E.s, 9 * NP);
+        if (cfg->is_diss) rc = rc ? rc : dalloc(c, &E.M, 18 * NP);
+        rc = rc ? rc : dalloc(c, &E.f, 3 * NP);
+        rc = rc ? rc : dalloc(c, &E.src_md, 9 * 125);
+        rc = rc ? rc : dalloc(c, &E.so_d, cfg->nt);
+        if (cfg->adjoint_flag == 2) rc = rc ? rc : dalloc(c, &E.grad, 5 * NP);
+        E.so.assign(cfg->nt, 0.f);
+    }
+    if (rc) {
+        ses3d_hip_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+void ses3d_hip_destroy(ses3d_hip_context *c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) hipFree(p);
+    for (auto &e : c->tev) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int ses3d_hip_set_model(ses3d_hip_context *c, const float *rhoinv, const float *mu, const float *lambda,
+                        const float *A, const float *B, const float *C, const float *Q) {
+    if (!c || !rhoinv || !mu || !lambda || !A || !B || !C) return fail(SES3D_HIP_EINVAL, "null model field");
+    if (c->cfg.is_diss && !Q) return fail(SES3D_HIP_EINVAL, "is_diss=1 needs Q");
+    std::vector<float> tmp(c->NP);
+    const float *src[7] = {rhoinv, mu, lambda, A, B, C, Q};
+    float *dst[7] = {c->rhoinv, c->mu, c->lam, c->A, c->B, c->C, c->Q};
+    for (int f = 0; f < 7; ++f) {
+        if (!dst[f] || !src[f]) continue;
+        ref_to_em(c, src[f], tmp.data());
+        TRY(upload(c, dst[f], tmp.data(), sizeof(float) * c->NP));
+    }
+    c->model_set = true;
+    c->initialised = false;
+    return 0;
+}
+
+int ses3d_hip_set_source(ses3d_hip_context *c, int32_t ev, int32_t source_type, float xxs, float yys,
+                         float zzs, const float *mom, const float *so) {
+    TRY(check_ev(c, ev));
+    if (!mom || !so) return fail(SES3D_HIP_EINVAL, "null source argument");
+    if (!(source_type == 1 || source_type == 2 || source_type == 3 || source_type == 10))
+        return fail(SES3D_HIP_EINVAL, "source_type must be 1, 2, 3 or 10");
+    EventHost &E = c->evh[ev];
+    E.source_type = source_type;
+    E.xxs = xxs; E.yys = yys; E.zzs = zzs;
+    for (int k = 0; k < 6; ++k) E.mom[k] = mom[k];
+    E.so.assign(so, so + c->cfg.nt);
+    E.have_source = true;
+    c->initialised = false;
+    return 0;
+}
+
+int ses3d_hip_set_receivers(ses3d_hip_context *c, int32_t ev, int32_t nr, const float *rec) {
+    TRY(check_ev(c, ev));
+    if (nr < 0 || (nr > 0 && !rec)) return fail(SES3D_HIP_EINVAL, "bad receiver list");
+    c->evh[ev].rec.assign(rec, rec + 3 * (size_t)nr);
+    c->initialised = false;
+    return 0;
+}
+
+int ses3d_hip_set_adjoint_sources(ses3d_hip_context *c, int32_t ev, int32_t n, const float *loc, const float *stf) {
+    TRY(check_ev(c, ev));
+    if (n < 0 || (n > 0 && (!loc || !stf))) return fail(SES3D_HIP_EINVAL, "bad adjoint source list");
+    c->evh[ev].ad_loc.assign(loc, loc + 3 * (size_t)n);
+    c->evh[ev].ad_stf.assign(stf, stf + (size_t)n * c->cfg.nt * 3);
+    c->initialised = false;
+    return 0;
+}
+
+int ses3d_hip_init(ses3d_hip_context *c) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (!c->model_set) return fail(SES3D_HIP_ESTATE, "set_model must precede init");
+    const auto &ms = c->ms;
+    const int nt = c->cfg.nt;
+    const int af = c->cfg.adjoint_flag;
+    // small per-axis tables (allocated once)
+    if (!c->Z) {
+        TRY(dalloc(c, &c->Z, ms.z.size()));
+        TRY(dalloc(c, &c->S, ms.sin_x.size()));
+        TRY(dalloc(c, &c->COT, ms.cot_x.size()));
+        TRY(dalloc(c, &c->IX, ms.IX.size()));
+        TRY(dalloc(c, &c->IY, ms.IY.size()));
+        TRY(dalloc(c, &c->IZ, ms.IZ.size()));
+        TRY(dalloc(c, &c->LUT, ms.LUT.size()));
+        TRY(dalloc(c, &c->w5, 5));
+    }
+    TRY(upload(c, c->Z, ms.z.data(), sizeof(float) * ms.z.size()));
+    TRY(upload(c, c->S, ms.sin_x.data(), sizeof(float) * ms.sin_x.size()));
+    TRY(upload(c, c->COT, ms.cot_x.data(), sizeof(float) * ms.cot_x.size()));
+    TRY(upload(c, c->IX, ms.IX.data(), sizeof(uint16_t) * ms.IX.size()));
+    TRY(upload(c, c->IY, ms.IY.data(), sizeof(uint16_t) * ms.IY.size()));
+    TRY(upload(c, c->IZ, ms.IZ.data(), sizeof(uint16_t) * ms.IZ.size()));
+    TRY(upload(c, c->LUT, ms.LUT.data(), sizeof(float) * ms.LUT.size()));
+    TRY(upload(c, c->w5, ms.w, sizeof(float) * 5));
+    MeshDev &M = c->mesh_h;
+    M.NX = c->NX; M.NY = c->NY; M.NZ = c->NZ; M.NBX = c->NBX; M.NBY = c->NBY; M.NBZ = c->NBZ;
+    M.NE = c->NE; M.NP = c->NP; M.NGB = c->NGB;
+    M.KX = ms.KX; M.KY = ms.KY; M.KZ = ms.KZ;
+    M.is_diss = c->cfg.is_diss;
+    M.dt = c->cfg.dt; M.Jac = ms.Jac;
+    M.sum_D_p = (c->cfg.D_p[0] + c->cfg.D_p[1]) + c->cfg.D_p[2];
+    for (int k = 0; k < 3; ++k) {
+        M.tau_p[k] = c->cfg.tau_p[k];
+        M.Lk[k] = (c->cfg.dt * c->cfg.D_p[k]) / c->cfg.tau_p[k];   // L=dt*D_p(k)/tau_p(k), :338
+    }
+    std::memcpy(M.tab + 0, ms.D1, sizeof(ms.D1));
+    std::memcpy(M.tab + 25, ms.D2, sizeof(ms.D2));
+    std::memcpy(M.tab + 50, ms.D3, sizeof(ms.D3));
+    std::memcpy(M.tab + 75, ms.WDL, sizeof(ms.WDL));
+    std::memcpy(M.tab + 100, ms.DUM1, sizeof(ms.DUM1));
+    std::memcpy(M.tab + 125, ms.DUM2, sizeof(ms.DUM2));
+    std::memcpy(M.tab + 150, ms.DUM3, sizeof(ms.DUM3));
+    M.lam = c->lam; M.mu = c->mu; M.A = c->A; M.B = c->B; M.C = c->C; M.Q = c->Q; M.rhoinv = c->rhoinv;
+    M.MM = c->MM; M.Z = c->Z; M.S = c->S; M.COT = c->COT; M.IX = c->IX; M.IY = c->IY; M.IZ = c->IZ;
+    M.LUT = c->LUT;
+    TRY(upload(c, c->mesh_d, &M, sizeof(MeshDev)));
+    // mass matrix
+    {
+        const long long threads = node_blocks(c) * 64;
+        hipLaunchKernelGGL(k_mass, dim3((unsigned)((threads + NODE_THREADS - 1) / NODE_THREADS)), dim3(NODE_THREADS), 0,
+                           c->stream, c->mesh_d, c->MM, c->w5);
+        HIP_TRY(hipGetLastError());
+    }
+    // events: zero fields, locate source / receivers / adjoint sources
+    std::vector<RecDev> recs;
+    std::vector<ForceGroup> groups;
+    std::vector<ForceDev> forces;
+    c->saving_vector.assign(nt + 1, 0);
+    for (int ev = 0; ev < c->n_events; ++ev) {
+        EventHost &E = c->evh[ev];
+        TRY(zero(c, E.v, sizeof(float) * 3 * c->NGB));
+        TRY(zero(c, E.d, sizeof(float) * 9 * c->NP));
+        TRY(zero(c, E.s, sizeof(float) * 9 * c->NP));
+        if (E.M) TRY(zero(c, E.M, sizeof(float) * 18 * c->NP));
+        if (E.grad) TRY(zero(c, E.grad, sizeof(double) * 5 * c->NP));
+        TRY(upload(c, E.so_d, E.so.data(), sizeof(float) * nt));
+        std::vector<float> md(9 * 125, 0.f);
+        E.sloc = ses3d::SourceLoc();
+        if ((af == 0 || af == 1) && E.have_source) {
+            E.sloc = ses3d::locate_source(ms, E.xxs, E.yys, E.zzs);
+            if (E.sloc.is_source && E.source_type == 10) ses3d::moment_weights(ms, E.sloc, E.mom, md.data());
+        }
+        TRY(upload(c, E.src_md, md.data(), sizeof(float) * md.size()));
+        // receivers
+        E.recs.clear();
+        if (af == 0 || af == 1) E.recs = ses3d::locate_receivers(ms, E.rec.data(), (int)(E.rec.size() / 3));
+        const int nr = (int)E.recs.size();
+        if (nr > 0) {
+            if (!E.seis) TRY(dalloc(c, &E.seis, (size_t)nr * 3 * nt));
+            if (!E.rec_w) TRY(dalloc(c, &E.rec_w, (size_t)nr * 125));
+            TRY(zero(c, E.seis, sizeof(float) * (size_t)nr * 3 * nt));
+            std::vector<float> w((size_t)nr * 125);
+            for (int r = 0; r < nr; ++r) {
+                ses3d::point_weights(E.recs[r].xl, E.recs[r].yl, E.recs[r].zl, &w[(size_t)r * 125]);
+                RecDev R;
+                R.ev = ev; R.ex = E.recs[r].ex; R.ey = E.recs[r].ey; R.ez = E.recs[r].ez;
+                R.w = E.rec_w + (size_t)r * 125;
+                R.out = E.seis + (size_t)r * 3 * nt;
+                recs.push_back(R);
+            }
+            TRY(upload(c, E.rec_w, w.data(), sizeof(float) * w.size()));
+        }
+        // point forces: forward single force or adjoint sources, grouped by element in order
+        std::vector<std::pair<long long, std::pair<int, std::vector<float>>>> plist;   // elem, (mask, w)
+        std::vector<std::vector<float>> pstf;
+        if ((af == 0 || af == 1) && E.have_source && E.sloc.is_source && E.source_type < 10) {
+            std::vector<float> w(125);
+            ses3d::point_weights(E.sloc.xl, E.sloc.yl, E.sloc.zl, w.data());
+            long long el = E.sloc.isx + (long long)c->NX * (E.sloc.isy + (long long)c->NY * E.sloc.isz);
+            std::vector<float> stf(3 * (size_t)nt, 0.f);
+            std::copy(E.so.begin(), E.so.end(), stf.begin() + (size_t)(E.source_type - 1) * nt);
+            plist.push_back({el, {1 << (E.source_type - 1), w}});
+            pstf.push_back(stf);
+        }
+        E.ads.clear();
+        if (af == 2) {
+            E.ads = ses3d::locate_adjoint_sources(ms, E.ad_loc.data(), (int)(E.ad_loc.size() / 3));
+            for (auto &a : E.ads) {
+                std::vector<float> w(125);
+                ses3d::point_weights(a.xl, a.yl, a.zl, w.data());
+                long long el = a.ex + (long long)c->NX * (a.ey + (long long)c->NY * a.ez);
+                std::vector<float> stf(3 * (size_t)nt);
+                for (int t = 0; t < nt; ++t)
+                    for (int cc = 0; cc < 3; ++cc) stf[(size_t)cc * nt + t] = E.ad_stf[((size_t)a.index * nt + t) * 3 + cc];
+                plist.push_back({el, {7, w}});
+                pstf.push_back(stf);
+            }
+        }
+        if (!plist.empty()) {
+            const size_t np = plist.size();
+            if (!E.force_w) TRY(dalloc(c, &E.force_w, np * 125));
+            if (!E.force_stf) TRY(dalloc(c, &E.force_stf, np * 3 * (size_t)nt));
+            std::vector<float> wall(np * 125), sall(np * 3 * (size_t)nt);
+            for (size_t k = 0; k < np; ++k) {
+                std::copy(plist[k].second.second.begin(), plist[k].second.second.end(), wall.begin() + k * 125);
+                std::copy(pstf[k].begin(), pstf[k].end(), sall.begin() + k * 3 * (size_t)nt);
+            }
+            TRY(upload(c, E.force_w, wall.data(), sizeof(float) * wall.size()));
+            TRY(upload(c, E.force_stf, sall.data(), sizeof(float) * sall.size()));
+            // group by element keeping source order within each element
+            std::map<long long, std::vector<size_t>> by_elem;
+            for (size_t k = 0; k < np; ++k) by_elem[plist[k].first].push_back(k);
+            for (auto &kv : by_elem) {
+                ForceGroup G;
+                G.ev = ev; G.elem = kv.first; G.first = (int)forces.size(); G.count = (int)kv.second.size();
+                for (size_t k : kv.second) {
+                    ForceDev F;
+                    F.comp_mask = plist[k].second.first;
+                    F.w = E.force_w + k * 125;
+                    F.stf = E.force_stf + k * 3 * (size_t)nt;
+                    forces.push_back(F);
+                }
+                groups.push_back(G);
+            }
+        }
+        // sampled-step slots: adjoint_flag 1 stores, 2 reads the fwd step nt+1-it
+        if (af == 1 || af == 2) {
+            TRY(build_slots(c, E, true));
+            std::vector<int> sl(nt + 1, -1);
+            for (int it = 1; it <= nt; ++it) {
+                if (af == 1) sl[it] = E.slot_h[it];
+                else sl[it] = E.slot_h[nt + 1 - it];
+            }
+            TRY(upload(c, E.slot_d, sl.data(), sizeof(int) * (nt + 1)));
+            for (int it = 1; it <= nt; ++it) c->saving_vector[it] = E.slot_h[it] >= 0 ? it : 0;
+        }
+    }
+    c->nrec_total = (int)recs.size();
+    if (!recs.empty()) {
+        if (c->recs_d) { /* reallocated below */ }
+        TRY(dalloc(c, &c->recs_d, recs.size()));
+        TRY(upload(c, c->recs_d, recs.data(), sizeof(RecDev) * recs.size()));
+    }
+    c->n_groups = (int)groups.size();
+    if (!groups.empty()) {
+        TRY(dalloc(c, &c->groups_d, groups.size()));
+        TRY(dalloc(c, &c->forces_d, forces.size()));
+        TRY(upload(c, c->groups_d, groups.data(), sizeof(ForceGroup) * groups.size()));
+        TRY(upload(c, c->forces_d, forces.data(), sizeof(ForceDev) * forces.size()));
+    }
+    TRY(refresh_events(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->initialised = true;
+    return 0;
+}
+
+int ses3d_hip_run(ses3d_hip_context *c, int32_t it_begin, int32_t n_steps) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (!c->initialised) return fail(SES3D_HIP_ESTATE, "init must precede run");
+    if (it_begin < 1 || n_steps < 0 || it_begin + n_steps - 1 > c->cfg.nt)
+        return fail(SES3D_HIP_EINVAL, "step range outside 1..nt");
+    const int af = c->cfg.adjoint_flag;
+    c->t_elem = c->t_node = 0; c->n_elem = 0;
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipEventRecord(t0, c->stream));
+    for (int it = it_begin; it < it_begin + n_steps; ++it) TRY(step(c, it, af, af != 2));
+    HIP_TRY(hipEventRecord(t1, c->stream));
+    if (c->profiling) {
+        HIP_TRY(hipEventSynchronize(t1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+        c->t_total = ms;
+    }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return 0;
+}
+
+int ses3d_hip_synchronize(ses3d_hip_context *c) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ses3d_hip_get_nr(ses3d_hip_context *c, int32_t ev, int32_t *nr) {
+    TRY(check_ev(c, ev));
+    if (!nr) return fail(SES3D_HIP_EINVAL, "null nr");
+    *nr = (int32_t)c->evh[ev].recs.size();
+    return 0;
+}
+
+int ses3d_hip_get_receiver_index(ses3d_hip_context *c, int32_t ev, int32_t *idx) {
+    TRY(check_ev(c, ev));
+    for (size_t r = 0; r < c->evh[ev].recs.size(); ++r) idx[r] = c->evh[ev].recs[r].index;
+    return 0;
+}
+
+int ses3d_hip_get_seismograms(ses3d_hip_context *c, int32_t ev, float *out) {
+    TRY(check_ev(c, ev));
+    EventHost &E = c->evh[ev];
+    const int nr = (int)E.recs.size(), nt = c->cfg.nt;
+    if (nr == 0) return 0;
+    std::vector<float> tmp((size_t)nr * 3 * nt);
+    TRY(download(c, tmp.data(), E.seis, sizeof(float) * tmp.size()));
+    for (int r = 0; r < nr; ++r)
+        for (int cc = 0; cc < 3; ++cc)
+            std::memcpy(out + ((size_t)cc * nr + r) * nt, &tmp[((size_t)r * 3 + cc) * nt], sizeof(float) * nt);
+    return 0;
+}
+
+int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, float *out) {
+    TRY(check_ev(c, ev));
+    if (!name || !out) return fail(SES3D_HIP_EINVAL, "null argument");
+    EventHost &E = c->evh[ev];
+    const std::string nm(name);
+    const long long NP = c->NP;
+    static const char *dn[9] = {"dxux", "dyuy", "dzuz", "dxuy", "dyux", "dxuz", "dzux", "dyuz", "dzuy"};
+    static const char *sn[9] = {"sxx", "syy", "szz", "sxy", "syx", "sxz", "szx", "syz", "szy"};
+    static const char *mn[6] = {"Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz"};
+    std::vector<float> tmp;
+    for (int k = 0; k < 9; ++k) {
+        if (nm == dn[k] || nm == sn[k]) {
+            tmp.resize(NP);
+            TRY(download(c, tmp.data(), (nm == dn[k] ? E.d : E.s) + k * NP, sizeof(float) * NP));
+            em_to_ref(c, tmp.data(), out);
+            return 0;
+        }
+    }
+    for (int k = 0; k < 6; ++k)
+        if (nm == mn[k]) {
+            if (!E.M) return fail(SES3D_HIP_EINVAL, "memory variables exist only when is_diss=1");
+            tmp.resize(NP);
+            for (int mech = 0; mech < 3; ++mech) {
+                TRY(download(c, tmp.data(), E.M + (k * 3 + mech) * NP, sizeof(float) * NP));
+                em_to_ref(c, tmp.data(), out + mech * NP);
+            }
+            return 0;
+        }
+    const int gx = 4 * c->NX + 1, gy = 4 * c->NY + 1, gz = 4 * c->NZ + 1;
+    auto global_out = [&](const float *dev) -> int {
+        std::vector<float> g(c->NGB);
+        TRY(download(c, g.data(), dev, sizeof(float) * c->NGB));
+        for (int z = 0; z < gz; ++z)
+            for (int y = 0; y < gy; ++y)
+                for (int x = 0; x < gx; ++x) out[x + (long long)gx * (y + (long long)gy * z)] = g[host_gidx(c, x, y, z)];
+        return 0;
+    };
+    auto local_out = [&](const float *dev) -> int {
+        std::vector<float> g(c->NGB);
+        TRY(download(c, g.data(), dev, sizeof(float) * c->NGB));
+        const long long NE = c->NE;
+        for (int n = 0; n < 5; ++n)
+            for (int m = 0; m < 5; ++m)
+                for (int l = 0; l < 5; ++l)
+                    for (int k = 0; k < c->NZ; ++k)
+                        for (int j = 0; j < c->NY; ++j)
+                            for (int i = 0; i < c->NX; ++i) {
+                                long long e = i + (long long)c->NX * (j + (long long)c->NY * k);
+                                out[e + NE * (l + 5 * (m + 5 * n))] = g[host_gidx(c, 4 * i + l, 4 * j + m, 4 * k + n)];
+                            }
+        return 0;
+    };
+    if (nm == "vx_global") return global_out(E.v);
+    if (nm == "vy_global") return global_out(E.v + c->NGB);
+    if (nm == "vz_global") return global_out(E.v + 2 * c->NGB);
+    if (nm == "MM_global") return global_out(c->MM);
+    // element-local velocity as the snapshots hold it: the local copy gathered at the
+    // start of the last step, i.e. the velocity before that step's update.  Between
+    // steps it equals a gather of the current global velocity of the previous step;
+    // here it is the gather of the current global velocity (the state after the last
+    // completed step), which is what the next step's gather will read.
+    if (nm == "vx") return local_out(E.v);
+    if (nm == "vy") return local_out(E.v + c->NGB);
+    if (nm == "vz") return local_out(E.v + 2 * c->NGB);
+    return fail(SES3D_HIP_EINVAL, "unknown field name " + nm);
+}
+
+int ses3d_hip_get_kernels(ses3d_hip_context *c, int32_t ev, double *g_rho, double *g_cp, double *g_csh,
+                          double *g_csv, double *g_Q) {
+    TRY(check_ev(c, ev));
+    EventHost &E = c->evh[ev];
+    if (!E.grad) return fail(SES3D_HIP_ESTATE, "kernels exist only for adjoint_flag=2 contexts");
+    const long long NP = c->NP;
+    std::vector<double> tmp(NP);
+    double *outs[5] = {g_rho, g_cp, g_csh, g_csv, g_Q};
+    for (int k = 0; k < 5; ++k) {
+        if (!outs[k]) continue;
+        if (k == 4 && !c->cfg.is_diss) return fail(SES3D_HIP_EINVAL, "grad_Q_mu needs is_diss=1");
+        TRY(download(c, tmp.data(), E.grad + k * NP, sizeof(double) * NP));
+        em_to_ref(c, tmp.data(), outs[k]);
+    }
+    if (g_Q && c->cfg.is_diss) {
+        // normalisation at it==nt (SOURCE/ses3d_grad_Q.f90:151-153): 2 mu K / Q
+        std::vector<float> mu(NP), q(NP), mur(NP), qr(NP);
+        TRY(download(c, mu.data(), c->mu, sizeof(float) * NP));
+        TRY(download(c, q.data(), c->Q, sizeof(float) * NP));
+        em_to_ref(c, mu.data(), mur.data());
+        em_to_ref(c, q.data(), qr.data());
+        for (long long i = 0; i < NP; ++i) g_Q[i] = ((2.0 * (double)mur[i]) * g_Q[i]) / (double)qr[i];
+    }
+    return 0;
+}
+
+int ses3d_hip_get_history(ses3d_hip_context *c, int32_t ev, int32_t step_, float *out9) {
+    TRY(check_ev(c, ev));
+    EventHost &E = c->evh[ev];
+    if (!E.hist || step_ < 1 || step_ > c->cfg.nt || E.slot_h.empty() || E.slot_h[step_] < 0)
+        return fail(SES3D_HIP_EINVAL, "step is not a stored forward step");
+    const long long NP = c->NP;
+    std::vector<float> tmp(NP);
+    for (int k = 0; k < 9; ++k) {
+        TRY(download(c, tmp.data(), E.hist + ((long long)E.slot_h[step_] * 9 + k) * NP, sizeof(float) * NP));
+        em_to_ref(c, tmp.data(), out9 + k * NP);
+    }
+    return 0;
+}
+
+int ses3d_hip_put_history(ses3d_hip_context *c, int32_t ev, int32_t step_, const float *in9) {
+    TRY(check_ev(c, ev));
+    EventHost &E = c->evh[ev];
+    if (!E.hist || step_ < 1 || step_ > c->cfg.nt || E.slot_h.empty() || E.slot_h[step_] < 0)
+        return fail(SES3D_HIP_EINVAL, "step is not a sampled forward step (init the adjoint context first)");
+    const long long NP = c->NP;
+    std::vector<float> tmp(NP);
+    for (int k = 0; k < 9; ++k) {
+        ref_to_em(c, in9 + k * NP, tmp.data());
+        TRY(upload(c, E.hist + ((long long)E.slot_h[step_] * 9 + k) * NP, tmp.data(), sizeof(float) * NP));
+    }
+    return 0;
+}
+
+int ses3d_hip_get_saving_vector(ses3d_hip_context *c, int32_t *sv) {
+    if (!c || !sv) return fail(SES3D_HIP_EINVAL, "null argument");
+    for (int it = 1; it <= c->cfg.nt; ++it) sv[it - 1] = c->saving_vector.empty() ? 0 : c->saving_vector[it];
+    return 0;
+}
+
+int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
+    (void)c; (void)k;
+    return fail(SES3D_HIP_ESTATE, "checkpointed forward not available in this build");
+}
+
+int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
+    (void)a; (void)f;
+    return fail(SES3D_HIP_ESTATE, "checkpointed adjoint not available in this build");
+}
+
+int ses3d_hip_memory_bytes(ses3d_hip_context *c, int64_t *bytes) {
+    if (!c || !bytes) return fail(SES3D_HIP_EINVAL, "null argument");
+    *bytes = c->bytes;
+    return 0;
+}
+
+int ses3d_hip_set_profiling(ses3d_hip_context *c, int32_t on) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    c->profiling = on != 0;
+    return 0;
+}
+
+int ses3d_hip_get_timing(ses3d_hip_context *c, double *total_ms, double *element_ms, double *node_ms,
+                         int32_t *n_element_launches) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (total_ms) *total_ms = c->t_total;
+    if (element_ms) *element_ms = c->t_elem;
+    if (node_ms) *node_ms = c->t_node;
+    if (n_element_launches) *n_element_launches = c->n_elem;
+    return 0;
+}
+
+}  // extern "C"

@@ -557,7 +557,7 @@ def write_ad_srcfile(path: str, locs_deg: np.ndarray) -> None:
     with open(path, "w") as fh:
         fh.write(f"{len(locs_deg)}\n")
         for a, b, c in locs_deg:
-            fh.write(f"{a!r} {b!r} {c!r}\n")
+            fh.write(f"{float(a)!r} {float(b)!r} {float(c)!r}\n")
 
 
 def read_ad_srcfile(path: str) -> np.ndarray:
@@ -585,10 +585,10 @@ def write_ad_src(path: str, loc_deg, adsrc_xyz_forward_time: np.ndarray) -> None
     with open(path, "w") as fh:
         fh.write("-- adjoint source ------------------\n")
         fh.write("-- source coordinates (colat,lon,depth)\n")
-        fh.write(f"{loc_deg[0]!r} {loc_deg[1]!r} {loc_deg[2]!r}\n")
+        fh.write(f"{float(loc_deg[0])!r} {float(loc_deg[1])!r} {float(loc_deg[2])!r}\n")
         fh.write("-- source time function (x, y, z) --\n")
         for k in range(nt - 1, -1, -1):
-            fh.write(f"{a[0, k]!r} {a[1, k]!r} {a[2, k]!r}\n")
+            fh.write(f"{float(a[0, k])!r} {float(a[1, k])!r} {float(a[2, k])!r}\n")
 
 
 def read_ad_src(path: str, nt: int) -> np.ndarray:

@@ -1,0 +1,93 @@
+"""Compact, self-contained SES3D test cases for the golden fixtures.
+
+A fixture stores every input value it needs (geometry, event, receivers, the
+source time function, the moment tensor and a 1D model as per-(k,n) vectors),
+so the run tree can be rebuilt bit for bit on any machine without
+regenerating random numbers or calling transcendental functions.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from ses3d_amd import formats as F, models as M
+
+CASES = {
+    # name: (nel, nt, dt, is_diss, source_type, src (colat, lon, depth), samp_ad, adjoint)
+    "elastic_mt": ((5, 6, 3), 60, 0.2, 0, 10, (50.2, 20.7, 25000.0), 3, False),
+    "anelastic_adjoint": ((5, 5, 4), 50, 0.2, 1, 10, (49.3, 19.1, 60000.0), 4, True),
+    "force_r": ((4, 5, 3), 40, 0.2, 0, 3, (51.0, 21.5, 15000.0), 3, False),
+}
+
+
+def compact_inputs(name: str, seed: int = 11) -> dict:
+    nel, nt, dt, is_diss, st, src, samp_ad, adj = CASES[name]
+    rng = np.random.default_rng(seed)
+    setup = M.section_setup(*nel, is_diss=bool(is_diss), adjoint_flag=1 if adj else 0, samp_ad=samp_ad)
+    box = F.make_boxes(setup)[0]
+    _, _, z = M.gll_coords(box)
+    rho, vp, vs = M.prem_like(z)
+    mu = rho * vs * vs
+    out = {
+        "nel": np.array(nel), "nt": nt, "dt": dt, "is_diss": is_diss, "source_type": st,
+        "src": np.array(src), "samp_ad": samp_ad, "adjoint": int(adj),
+        "moment": np.round(M.random_moment_tensor(rng) / 1e15) * 1e15,
+        "stf": M.ricker(nt + 8, dt, 8.0, t0=6.0),
+        "rhoinv_z": (1.0 / rho).astype(np.float32),
+        "mu_z": mu.astype(np.float32),
+        "lambda_z": (rho * vp * vp - 2.0 * mu).astype(np.float32),
+        "Q_z": M.q_mu_by_depth(z).astype(np.float32),
+    }
+    rec = M.surface_receiver_grid(setup, 2, 2)
+    out["rec"] = np.stack([rec.colat, rec.lon, rec.depth], 1)
+    return out
+
+
+def write_tree(fx: dict, root: str, adjoint_flag: int = None) -> None:
+    """Write the run tree for a (loaded or fresh) compact fixture."""
+    nel = tuple(int(v) for v in fx["nel"])
+    nt, dt = int(fx["nt"]), float(fx["dt"])
+    is_diss = int(fx["is_diss"])
+    adj = int(fx["adjoint"])
+    af = (1 if adj else 0) if adjoint_flag is None else adjoint_flag
+    setup = M.section_setup(*nel, is_diss=bool(is_diss), adjoint_flag=af, samp_ad=int(fx["samp_ad"]))
+    box = F.make_boxes(setup)[0]
+    shape = (box.nx + 1, box.ny + 1, box.nz + 1, 5, 5, 5)
+
+    def bz(v):
+        return np.asfortranarray(np.broadcast_to(np.asarray(v, np.float32)[None, None, :, None, None, :], shape))
+
+    model = {"rhoinv": bz(fx["rhoinv_z"]), "mu": bz(fx["mu_z"]), "lambda": bz(fx["lambda_z"]),
+             "A": np.zeros(shape, np.float32, order="F"), "B": np.zeros(shape, np.float32, order="F"),
+             "C": np.zeros(shape, np.float32, order="F")}
+    if is_diss:
+        model["Q"] = bz(fx["Q_z"])
+    src = [float(v) for v in fx["src"]]
+    ev = F.Event(nt=nt, dt=dt, xxs=src[0], yys=src[1], zzs=src[2], source_type=int(fx["source_type"]),
+                 moment=tuple(float(v) for v in fx["moment"]), ofd="../DATA/OUTPUT/1/", ssamp=nt,
+                 output_displacement=1)
+    r = np.asarray(fx["rec"], np.float64)
+    rec = F.Receivers([f"R{k:03d}.GG" for k in range(len(r))], r[:, 0], r[:, 1], r[:, 2])
+    case = M.RunCase(setup, [ev], [rec], np.asarray(fx["stf"], np.float32), model)
+    M.write_run_tree(root, case)
+
+
+def adjoint_inputs(fx: dict, seis: np.ndarray) -> tuple:
+    """Adjoint sources at the receivers: an L2-type residual built from the
+    forward seismograms (3, nr, nt), written in forward time, scaled."""
+    r = np.asarray(fx["rec"], np.float64)
+    adsrc = np.stack([seis[:, k, :].astype(np.float64) * 1e6 for k in range(len(r))])
+    return r, adsrc
+
+
+def write_adjoint(fx: dict, root: str, seis: np.ndarray) -> None:
+    locs, adsrc = adjoint_inputs(fx, seis)
+    d = os.path.join(root, "ADJOINT", "1")
+    os.makedirs(d, exist_ok=True)
+    F.write_ad_srcfile(os.path.join(d, "ad_srcfile"), locs)
+    for k in range(len(locs)):
+        F.write_ad_src(os.path.join(d, f"ad_src_{k + 1}"), locs[k], adsrc[k])
+    s = F.Setup.read(os.path.join(root, "INPUT", "setup"))
+    s.adjoint_flag = 2
+    s.write(os.path.join(root, "INPUT", "setup"))

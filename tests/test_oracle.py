@@ -1,0 +1,146 @@
+"""Pins of the CPU oracle (oracle/ses3d_oracle.c).
+
+1. Against the reference: the golden fixtures hold outputs of the reference
+   Fortran solver compiled from its own sources (tests/golden/make_golden.py);
+   the float32 oracle must reproduce them bit for bit (seismograms, velocity
+   snapshots, stored forward fields, saving vector, the four elastic kernels).
+2. Against mathematics, for what the fixtures leave open: GLL knots and
+   weights, the Lagrange derivative matrix, the mass matrix against the closed
+   form of the section's mass, and the float64 oracle agreeing with the float32
+   one to float32 rounding.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from ses3d_amd import formats as F, inputs as I
+from tests.golden import fixture_case as FC
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return dict(np.load(os.path.join(GOLDEN, f"{name}.npz")))
+
+
+def _oracle(run, e, af, precision=32):
+    from tests.cases import oracle_for
+    return oracle_for(run, e, adjoint_flag=af, precision=precision)
+
+
+@pytest.mark.parametrize("name", ["elastic_mt", "force_r", "anelastic_adjoint"])
+def test_oracle_matches_reference_fixture(name, oracle_lib, tmp_path):
+    fx = _load(name)
+    FC.write_tree(fx, str(tmp_path))
+    run = I.load_run(str(tmp_path))
+    e = run.events[0]
+    o = _oracle(run, e, run.adjoint_flag)
+    nt = e.nt
+    o.run(1, nt - 1)
+    # the snapshot of step nt holds the velocity gathered at the start of step nt
+    vg = {c: o.get(c + "g").astype(np.float32) for c in ("vx", "vy", "vz")}
+    o.run(nt, 1)
+    seis = o.get("seismograms").astype(np.float32)
+    np.testing.assert_array_equal(seis, fx["seismograms"])
+    for c in ("vx", "vy", "vz"):
+        np.testing.assert_array_equal(o.get(c).astype(np.float32), fx[c])
+    if int(fx["adjoint"]):
+        np.testing.assert_array_equal(o.saving_vector(), fx["saving_vector"])
+        for tag in ("first", "last"):
+            np.testing.assert_array_equal(o.history(int(fx[f"history_{tag}_step"])), fx[f"history_{tag}"])
+        FC.write_adjoint(fx, str(tmp_path), seis)
+        run2 = I.load_run(str(tmp_path))
+        oa = _oracle(run2, run2.events[0], 2)
+        oa.run(1, nt, fwd=o)
+        for g in ("grad_rho", "grad_cp", "grad_csh", "grad_csv"):
+            np.testing.assert_array_equal(oa.get(g).astype(np.float32), fx[g])
+        np.testing.assert_array_equal(oa.get("vx").astype(np.float32), fx["adjoint_vx"])
+
+
+def test_gll_tables(oracle_lib):
+    """Knots are the GLL points of degree 4 (roots of (1-x^2) P4'(x)), weights
+    2/(n(n+1) P4(x)^2); dl(i,j) = L_i'(x_j) differentiates every polynomial of
+    degree <= 4 exactly; Lagrange basis is a partition of unity."""
+    from oracle import pyoracle as PO
+    x = np.array([-1.0, -np.sqrt(3.0 / 7.0), 0.0, np.sqrt(3.0 / 7.0), 1.0])
+    P4 = np.polynomial.legendre.Legendre.basis(4)
+    w_exact = 2.0 / (20.0 * P4(x) ** 2)
+    o = PO.Oracle(nx=1, ny=1, nz=1, xmin=0.7, xmax=0.8, ymin=0.2, ymax=0.3, zmin=6.0e6, zmax=6.1e6,
+                  zmax_global=6.1e6, nt=2, dt=0.1, precision=64)
+    o.set_model({k: np.ones((2, 2, 2, 5, 5, 5), np.float32) for k in ("rhoinv", "mu", "lambda", "A", "B", "C")})
+    o.set_source(10, 0.75, 0.25, 50000.0, np.zeros(6, np.float32), np.zeros(2, np.float32))
+    o.set_receivers(np.zeros((0, 3), np.float32))
+    o.init()
+    sv = o.setup_values()
+    # reference weight literals are truncated to 13 decimals (ses3d_init.f90:135-139)
+    np.testing.assert_allclose(sv["w"], w_exact, rtol=0, atol=1e-12)
+    D = sv["dl"]   # D[i, j] = L_i'(x_j)
+    for deg in range(5):
+        f = x ** deg
+        df = deg * x ** (deg - 1) if deg else np.zeros(5)
+        np.testing.assert_allclose(f @ D, df, atol=1e-12)
+    np.testing.assert_allclose(D.sum(axis=0), 0.0, atol=1e-12)
+    for xv in (-0.9, -0.3, 0.1, 0.77):
+        s = sum(PO.lgll(i, xv, 64) for i in range(5))
+        assert abs(s - 1.0) < 1e-12
+        for i in range(5):
+            # L_i(x_j) = delta_ij
+            assert abs(PO.lgll(i, x[i], 64) - 1.0) < 1e-12
+
+
+def test_mass_matrix_closed_form(oracle_lib):
+    """Sum of the (negative) global mass equals -rho * volume of the section,
+    -rho (r2^3 - r1^3)/3 (cos t1 - cos t2)(p2 - p1), up to GLL quadrature error
+    of sin(theta) r^2 (ses3d_init.f90:254-277)."""
+    from oracle import pyoracle as PO
+    t1, t2, p1, p2, r1, r2 = 0.70, 0.90, 0.2, 0.45, 5.4e6, 6.371e6
+    rho = 3300.0
+    o = PO.Oracle(nx=3, ny=4, nz=2, xmin=t1, xmax=t2, ymin=p1, ymax=p2, zmin=r1, zmax=r2, zmax_global=r2,
+                  nt=2, dt=0.1, precision=64)
+    shape = (4, 5, 3, 5, 5, 5)
+    m = {k: np.zeros(shape, np.float32) for k in ("mu", "lambda", "A", "B", "C")}
+    m["rhoinv"] = np.full(shape, 1.0 / rho, np.float32)
+    o.set_model(m)
+    o.set_source(10, 0.8, 0.3, 1e4, np.zeros(6, np.float32), np.zeros(2, np.float32))
+    o.set_receivers(np.zeros((0, 3), np.float32))
+    o.init()
+    total = o.get("MM_global").sum()
+    exact = -rho * (r2 ** 3 - r1 ** 3) / 3.0 * (np.cos(t1) - np.cos(t2)) * (p2 - p1)
+    assert abs(total / exact - 1.0) < 1e-6
+
+
+def test_float64_oracle_tracks_float32(oracle_lib, tmp_path):
+    """The float64 restatement agrees with the float32 one to float32 rounding
+    growth only (relative L2 of the seismograms < 1e-5): a dropped term or a
+    wrong sign in either would show up as an O(1) difference."""
+    from tests.cases import make_case, oracle_for
+    run = make_case(str(tmp_path), nel=(5, 5, 4), nt=60, is_diss=True, seed=5)
+    e = run.events[0]
+    o32 = oracle_for(run, e, precision=32)
+    o64 = oracle_for(run, e, precision=64)
+    o32.run(1, e.nt)
+    o64.run(1, e.nt)
+    a, b = o32.get("seismograms"), o64.get("seismograms")
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert np.linalg.norm(b) > 0 and rel < 1e-5, rel
+
+
+def test_reference_binary_live(oracle_lib, tmp_path):
+    """When the reference sources are present, rebuild the reference and check
+    one fresh case against the oracle (skips where the reference is absent)."""
+    from oracle import pyoracle as PO
+    if not os.path.isdir("/root/reference/SOURCE") or not PO.build_ref():
+        pytest.skip("reference sources not present")
+    from tests.cases import make_case, oracle_for
+    run = make_case(str(tmp_path), nel=(5, 6, 3), nt=30, is_diss=True, seed=21, n_rec=(2, 3))
+    r = PO.run_ref(str(tmp_path))
+    assert r.returncode == 0, r.stderr[-1000:]
+    e = run.events[0]
+    o = oracle_for(run, e)
+    o.run(1, e.nt)
+    ofd = os.path.join(str(tmp_path), "MAIN", e.event.ofd)
+    seis = o.get("seismograms").astype(np.float32)
+    for k, n in enumerate(e.receivers.names):
+        np.testing.assert_array_equal(F.read_seismogram(ofd, n)["trace"].astype(np.float32), seis[:, k])
