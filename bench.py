@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: GLL-point updates per second of the SES3D forward time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload anelastic_96x96x48]
+
+A step is one iteration of the reference time loop (SOURCE/ses3d_main.f90:184-265)
+over the whole section: element kernel, point forces, node kernel and the
+seismogram sampling of 200 receivers.  Inputs are resident in HBM before the
+timed region.  updates/s = N_elements * 125 * K / wall.
+
+The JSON line also carries
+  roofline      the element kernel (the dominant kernel): algorithmic bytes per
+                launch / mean launch duration measured with HIP events on the
+                kernel's stream over the timed region, against 8 TB/s HBM;
+  cpu_baseline  the reference Fortran solver (oracle/_ref, compiled from the
+                reference sources) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GLL-pt updates/s fwd; s per event-gradient, batched vs plain, 1 MI355X"
+HBM_PEAK = 8.0e12
+
+
+def algorithmic_bytes(nel, is_diss: bool) -> dict:
+    """Algorithmic HBM bytes per GLL-point update (float32).
+
+    Whole step: state read+written once (9 displacement gradients, 9 split
+    stresses, 18 memory variables if anelastic), coefficients read once
+    (lambda, mu, A, B, C [, Q]), velocity read+written and the mass read once
+    per global node (rho_g = global nodes / GLL points).  Element kernel: all
+    of it except the velocity write and the mass read (node kernel)."""
+    nx, ny, nz = nel
+    n_p = nx * ny * nz * 125
+    n_g = (4 * nx + 1) * (4 * ny + 1) * (4 * nz + 1)
+    rho_g = n_g / n_p
+    state = 18 + 18 + (36 if is_diss else 0)
+    coef = 6 if is_diss else 5
+    step = 4 * (state + coef + 7 * rho_g)
+    elem = 4 * (state + coef + 3 * rho_g)
+    return {"rho_g": rho_g, "step": step, "element": elem, "node": 4 * 4 * rho_g}
+
+
+def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
+    """Time the reference binary on its largest one-rank mesh (its static
+    arrays hold 23 x 28 x 8 elements, SOURCE/ses3d_modules.f90:13-15);
+    per-step cost = difference of two runs of different length."""
+    from oracle import pyoracle as PO
+    from ses3d_amd import formats as F, models as M
+    if not PO.ref_available():
+        return None
+    nel = (23, 28, 8)
+    times = []
+    for nt in steps:
+        root = tempfile.mkdtemp(prefix="ses3d_cpu_")
+        try:
+            setup = M.section_setup(*nel, is_diss=is_diss)
+            box = F.make_boxes(setup)[0]
+            pert = M.gaussian_random_field(box, 1, corr_len=200e3, rms=0.03)
+            model = M.make_model(box, is_diss=is_diss, perturbation=pert)
+            rng = np.random.default_rng(0)
+            ev = F.Event(nt=nt, dt=0.1, xxs=50.0, yys=20.0, zzs=10000.0, source_type=10,
+                         moment=tuple(M.random_moment_tensor(rng)), ofd="../DATA/OUTPUT/1/", ssamp=10 ** 6)
+            rec = M.surface_receiver_grid(setup, 10, 10)
+            M.write_run_tree(root, M.RunCase(setup, [ev], [rec], M.ricker(nt + 4, 0.1, 20.0), model))
+            t0 = time.perf_counter()
+            r = PO.run_ref(root, timeout=timeout)
+            t1 = time.perf_counter()
+            if r.returncode != 0:
+                return None
+            times.append(t1 - t0)
+        finally:
+            shutil.rmtree(root, ignore_errors=True)
+    n_p = int(np.prod(nel)) * 125
+    per_step = (times[1] - times[0]) / (steps[1] - steps[0])
+    return {"value": n_p / per_step, "unit": "GLL-pt updates/s", "cores": 1, "kind": "reference",
+            "sample": (f"reference Fortran (amdflang -O3, 1 MPI rank) on {nel[0]}x{nel[1]}x{nel[2]} "
+                       f"{'anelastic' if is_diss else 'elastic'} elements (its compile-time maximum per "
+                       f"rank), per-step time = difference of {steps[1]}- and {steps[0]}-step runs "
+                       f"({times[1]:.1f} s and {times[0]:.1f} s)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="anelastic_96x96x48")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run a device copy of this many MiB (for HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the whole section is resident on a single MI355X and this "
+              "build runs no multi-GPU path (--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": 4 * n, "launches": 5}))
+        return
+
+    from ses3d_amd import configs
+    from ses3d_amd.solver import Solver
+
+    w = configs.WORKLOADS[args.workload]
+    nt = args.warmup + args.steps
+    t0 = time.perf_counter()
+    run = configs.build(w, seed=0, nt=nt)
+    print(f"# built {w.name} ({w.n_points} GLL points) in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    s = Solver.from_run(run, adjoint_flag=0, pml=w.pml, nt=nt)
+    s.set_model(run.model)
+    s.load_event(0, run.events[0])
+    s.init()
+    del run
+    print(f"# initialised; device memory {s.memory_bytes() / 2**30:.2f} GiB", file=sys.stderr, flush=True)
+    if args.warmup:
+        s.run(1, args.warmup)
+    s.set_profiling(True)
+    torch.cuda.synchronize()
+    s.synchronize()
+    t1 = time.perf_counter()
+    s.run(args.warmup + 1, args.steps, sync=False)
+    s.synchronize()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    tm = s.timing()
+    seis = s.seismograms()
+    if not np.isfinite(seis).all():
+        raise RuntimeError("non-finite seismograms")
+    wall = t2 - t1
+    n_p = w.n_points
+    value = n_p * args.steps / wall
+    ab = algorithmic_bytes(w.nel, w.is_diss)
+    elem_ms = tm["element_ms"] / max(tm["n_element"], 1)
+    achieved = ab["element"] * n_p / (elem_ms * 1e-3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"k_element_{w.name}_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC, "value": value, "unit": "GLL-pt updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain forward {w.name} (lpd=4, {n_p} GLL points, "
+                               f"{'anelastic 3 mechanisms' if w.is_diss else 'elastic'}, pml={w.pml}, "
+                               f"dt={w.dt}, 1 event, {s.nr(0)} receivers)",
+                   "elements": list(w.nel), "gll_points": n_p, "timed_steps_from": args.warmup + 1},
+        "roofline": {"bound": "hbm", "kernel": "k_element", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "algorithmic_bytes_per_point": ab["element"], "points_per_launch": n_p,
+                     "mean_launch_ms": elem_ms},
+        "step_roofline_frac": value * ab["step"] / HBM_PEAK,
+        "kernel_ms_per_step": {"element": elem_ms, "node": tm["node_ms"] / max(tm["n_element"], 1)},
+    }
+    s.close()
+    if not args.no_cpu_baseline:
+        print("# timing the reference CPU solver", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_reference(w.is_diss)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

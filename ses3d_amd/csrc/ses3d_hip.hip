@@ -702,7 +702,9 @@ int launch_node(ses3d_hip_context *c) {
     return 0;
 }
 
-// One iteration of the reference time loop for all events.
+// One iteration of the reference time loop for all events.  With profiling on,
+// HIP events bracket the element and node kernels on the context's stream;
+// they are read back after the run, so the stream is never stalled.
 int step(ses3d_hip_context *c, int it, int mode, bool record) {
     const int nt = c->cfg.nt;
     if (record && c->nrec_total > 0) {
@@ -711,25 +713,42 @@ int step(ses3d_hip_context *c, int it, int mode, bool record) {
                            c->ev_d, c->recs_d, c->nrec_total, nt, it);
         HIP_TRY(hipGetLastError());
     }
-    if (c->profiling) HIP_TRY(hipEventRecord(c->tev[0], c->stream));
+    hipEvent_t *ev = nullptr;
+    if (c->profiling) {
+        const size_t need = 4 * (size_t)(c->n_elem + 1);
+        while (c->tev.size() < need) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            c->tev.push_back(e);
+        }
+        ev = &c->tev[4 * (size_t)c->n_elem];
+        HIP_TRY(hipEventRecord(ev[0], c->stream));
+    }
     TRY(launch_element(c, it, mode));
-    if (c->profiling) HIP_TRY(hipEventRecord(c->tev[1], c->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
     if (c->n_groups > 0) {
         hipLaunchKernelGGL(k_forces, dim3(c->n_groups), dim3(128), 0, c->stream, c->mesh_d, c->ev_d,
                            c->groups_d, c->forces_d, nt, it);
         HIP_TRY(hipGetLastError());
     }
-    if (c->profiling) HIP_TRY(hipEventRecord(c->tev[2], c->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], c->stream));
     TRY(launch_node(c));
-    if (c->profiling) {
-        HIP_TRY(hipEventRecord(c->tev[3], c->stream));
-        HIP_TRY(hipEventSynchronize(c->tev[3]));
+    if (ev) {
+        HIP_TRY(hipEventRecord(ev[3], c->stream));
+        c->n_elem += 1;
+    }
+    return 0;
+}
+
+int collect_timing(ses3d_hip_context *c) {
+    if (!c->profiling) return 0;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->n_elem; ++k) {
         float a = 0, b = 0;
-        HIP_TRY(hipEventElapsedTime(&a, c->tev[0], c->tev[1]));
-        HIP_TRY(hipEventElapsedTime(&b, c->tev[2], c->tev[3]));
+        HIP_TRY(hipEventElapsedTime(&a, c->tev[4 * k], c->tev[4 * k + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, c->tev[4 * k + 2], c->tev[4 * k + 3]));
         c->t_elem += a;
         c->t_node += b;
-        c->n_elem += 1;
     }
     return 0;
 }
@@ -801,8 +820,6 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
         delete c;
         return fail(SES3D_HIP_EDEVICE, "hipStreamCreate failed");
     }
-    c->tev.resize(4);
-    for (auto &e : c->tev) hipEventCreate(&e);
     c->evh.resize(n_events);
     c->ms.build(cfg->nx, cfg->ny, cfg->nz, cfg->pml, cfg->xmin, cfg->xmax, cfg->ymin, cfg->ymax, cfg->zmin,
                 cfg->zmax, cfg->zmax_global, cfg->dt);
@@ -1080,21 +1097,24 @@ int ses3d_hip_run(ses3d_hip_context *c, int32_t it_begin, int32_t n_steps) {
     if (it_begin < 1 || n_steps < 0 || it_begin + n_steps - 1 > c->cfg.nt)
         return fail(SES3D_HIP_EINVAL, "step range outside 1..nt");
     const int af = c->cfg.adjoint_flag;
-    c->t_elem = c->t_node = 0; c->n_elem = 0;
-    hipEvent_t t0, t1;
-    HIP_TRY(hipEventCreate(&t0));
-    HIP_TRY(hipEventCreate(&t1));
-    HIP_TRY(hipEventRecord(t0, c->stream));
-    for (int it = it_begin; it < it_begin + n_steps; ++it) TRY(step(c, it, af, af != 2));
-    HIP_TRY(hipEventRecord(t1, c->stream));
+    c->t_elem = c->t_node = c->t_total = 0;
+    c->n_elem = 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
     if (c->profiling) {
-        HIP_TRY(hipEventSynchronize(t1));
+        HIP_TRY(hipEventCreate(&t0));
+        HIP_TRY(hipEventCreate(&t1));
+        HIP_TRY(hipEventRecord(t0, c->stream));
+    }
+    for (int it = it_begin; it < it_begin + n_steps; ++it) TRY(step(c, it, af, af != 2));
+    if (c->profiling) {
+        HIP_TRY(hipEventRecord(t1, c->stream));
+        TRY(collect_timing(c));
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
         c->t_total = ms;
+        hipEventDestroy(t0);
+        hipEventDestroy(t1);
     }
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
     return 0;
 }
 

@@ -99,20 +99,36 @@ def _cartesian(colat, lon, r):
     return r * st * np.cos(lon), r * st * np.sin(lon), r * np.cos(colat)
 
 
+def _lin_interp_matrix(xq: np.ndarray, xg: np.ndarray) -> np.ndarray:
+    """Dense (len(xq), len(xg)) matrix of 1D linear interpolation weights
+    (zero outside the grid)."""
+    W = np.zeros((len(xq), len(xg)))
+    h = xg[1] - xg[0]
+    t = (xq - xg[0]) / h
+    i0 = np.clip(np.floor(t).astype(int), 0, len(xg) - 2)
+    f = t - i0
+    ok = (t >= 0) & (t <= len(xg) - 1)
+    rows = np.arange(len(xq))
+    W[rows[ok], i0[ok]] = 1.0 - f[ok]
+    W[rows[ok], i0[ok] + 1] = f[ok]
+    return W
+
+
 def gaussian_random_field(box: F.Box, seed: int, corr_len: float = 200e3, rms: float = 0.03,
                           spacing: float = 40e3):
     """Seeded Gaussian-correlated random field on the box's GLL points.
 
-    White noise on a regular grid in (arc-length θ, arc-length φ, depth) with
-    ``spacing`` metres, filtered by a Gaussian kernel of correlation length
-    ``corr_len`` in Fourier space, scaled to ``rms``, then interpolated
-    trilinearly to every GLL point.  Returns an array in reference layout.
+    White noise on a regular grid in (arc length along theta, arc length
+    along phi at mid-colatitude, depth) with ``spacing`` metres, filtered by
+    a Gaussian kernel of correlation length ``corr_len`` in Fourier space,
+    scaled to ``rms``, then interpolated trilinearly to the GLL points.  The
+    GLL points form a tensor grid of per-axis coordinates, so the
+    interpolation is applied axis by axis.  Returns reference layout.
     """
-    from scipy.interpolate import RegularGridInterpolator
-
     rng = np.random.default_rng(seed)
+    smid = np.sin(0.5 * (box.xmin + box.xmax))
     lx = (box.xmax - box.xmin) * EARTH_R
-    ly = (box.ymax - box.ymin) * EARTH_R * np.sin(0.5 * (box.xmin + box.xmax))
+    ly = (box.ymax - box.ymin) * EARTH_R * smid
     lz = box.zmax - box.zmin
     pad = 2 * corr_len
     n = [max(4, int(np.ceil((L + 2 * pad) / spacing)) + 1) for L in (lx, ly, lz)]
@@ -123,14 +139,16 @@ def gaussian_random_field(box: F.Box, seed: int, corr_len: float = 200e3, rms: f
     f = np.real(np.fft.ifftn(np.fft.fftn(noise) * filt))
     f *= rms / max(f.std(), 1e-30)
     ax = [np.arange(m) * spacing - pad for m in n]
-    interp = RegularGridInterpolator(ax, f, bounds_error=False, fill_value=0.0)
     x, y, z = gll_coords(box)
-    X, Y, Z = _broadcast(box, x, y, z)
-    sx = (X - box.xmin) * EARTH_R
-    sy = (Y - box.ymin) * EARTH_R * np.sin(0.5 * (box.xmin + box.xmax))
-    sz = box.zmax - Z
-    pts = np.stack([sx.ravel(order="F"), sy.ravel(order="F"), sz.ravel(order="F")], axis=1)
-    return interp(pts).reshape(X.shape, order="F")
+    Wx = _lin_interp_matrix(((x - box.xmin) * EARTH_R).ravel(), ax[0])
+    Wy = _lin_interp_matrix(((y - box.ymin) * EARTH_R * smid).ravel(), ax[1])
+    Wz = _lin_interp_matrix((box.zmax - z).ravel(), ax[2])
+    g = np.einsum("ia,abc->ibc", Wx, f)
+    g = np.einsum("jb,ibc->ijc", Wy, g)
+    g = np.einsum("kc,ijc->ijk", Wz, g)          # (NX*5, NY*5, NZ*5)
+    NX, NY, NZ = x.shape[0], y.shape[0], z.shape[0]
+    g = g.reshape(NX, 5, NY, 5, NZ, 5).transpose(0, 2, 4, 1, 3, 5)
+    return np.asfortranarray(g)
 
 
 def gaussian_blobs(box: F.Box, seed: int, n_blobs: int = 6, radius: float = 300e3, amp: float = 0.02):
@@ -165,29 +183,25 @@ def make_model(box: F.Box, is_diss: bool = False, perturbation: Optional[np.ndar
     the one the reference uses (SOURCE/ses3d_init.f90:47-54).
     """
     x, y, z = gll_coords(box)
-    _, _, Z = _broadcast(box, x, y, z)
-    rho, vp, vs = prem_like(Z)
+    shape = (x.shape[0], y.shape[0], z.shape[0], 5, 5, 5)
+    rho1, vp1, vs1 = prem_like(z)                       # (NZ, 5)
+    bz = lambda a: np.broadcast_to(a[None, None, :, None, None, :], shape)
     if perturbation is not None:
         dlnvp = dlnvs = dlnrho = perturbation
-    if dlnvp is not None:
-        vp = vp * (1.0 + dlnvp)
-    if dlnvs is not None:
-        vs = vs * (1.0 + dlnvs)
-    if dlnrho is not None:
-        rho = rho * (1.0 + dlnrho)
+    rho = bz(rho1) if dlnrho is None else bz(rho1) * (1.0 + dlnrho)
+    vp = bz(vp1) if dlnvp is None else bz(vp1) * (1.0 + dlnvp)
+    vs = bz(vs1) if dlnvs is None else bz(vs1) * (1.0 + dlnvs)
     mu = rho * vs * vs
-    lam = rho * vp * vp - 2.0 * mu
-    shape = Z.shape
     out = {
         "rhoinv": (1.0 / rho).astype(np.float32),
         "mu": mu.astype(np.float32),
-        "lambda": lam.astype(np.float32),
+        "lambda": (rho * vp * vp - 2.0 * mu).astype(np.float32),
         "A": np.zeros(shape, np.float32),
         "B": np.zeros(shape, np.float32),
         "C": np.zeros(shape, np.float32),
     }
     if is_diss:
-        out["Q"] = q_mu_by_depth(Z).astype(np.float32)
+        out["Q"] = bz(q_mu_by_depth(z)).astype(np.float32)
     return {k: np.asfortranarray(v) for k, v in out.items()}
 
 
