@@ -69,6 +69,7 @@ struct MeshDev {
     const float *Z, *S, *COT;
     const unsigned short *IX, *IY, *IZ;
     const float *LUT;
+    const unsigned char *PMLX, *PMLY, *PMLZ;   // per element index: inside an absorbing layer
 };
 
 struct EventDev {
@@ -86,6 +87,29 @@ struct EventDev {
     float sdt;             // samp_ad*dt
 };
 
+// Loads and stores of the streamed per-point state (touched once per step).
+// SES3D_NT_LOAD / SES3D_NT_STORE select non-temporal cache hints for them.
+#ifndef SES3D_NT_LOAD
+#define SES3D_NT_LOAD 0
+#endif
+#ifndef SES3D_NT_STORE
+#define SES3D_NT_STORE 0
+#endif
+__device__ __forceinline__ float ldnt(const float *p) {
+#if SES3D_NT_LOAD
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+__device__ __forceinline__ void stnt(float *p, float v) {
+#if SES3D_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 __device__ __forceinline__ long long gidx(int gx, int gy, int gz, int NBX, int NBY) {
     return ((((long long)(gz >> 2) * NBY + (gy >> 2)) * NBX + (gx >> 2)) << 6) +
            (((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
@@ -102,8 +126,17 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // element kernel
 // MODE 0: forward; 1: forward, store sampled fields when slot[it] >= 0;
 // 2: adjoint, accumulate kernels when slot[it] >= 0.
+// ANISO 0: the model has A = B = C = 0 everywhere (checked on the host), so
+// their terms, exact zeros in the reference's sums, are not formed.
+//
+// Elements outside every absorbing layer have prof_x = prof_y = prof_z = 0 and
+// taper 1 at all their nodes.  There the split stresses of each pair
+// (sxy,syx), (sxz,szx), (syz,szy) obey the same update from the same zero
+// start, so they are bit-identical and only one of each pair is stored; and in
+// forward mode (MODE 0) the displacement gradient feeds nothing but the
+// absorbing terms, so it is integrated only inside the layers.
 // ---------------------------------------------------------------------------
-template <int DISS, int MODE>
+template <int DISS, int MODE, int ANISO>
 __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restrict__ mesh,
                                                           const EventDev *__restrict__ evs, int it,
                                                           int blocks_per_event) {
@@ -114,8 +147,9 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
     const EventDev &E = evs[ev];
     const int t = threadIdx.x;
     const int sub = t >> 7, nd = t & (TPE - 1);
-    const long long e = (long long)eb * EPB + sub;
-    const long long NE = Ms.NE, NP = Ms.NP;
+    const int NE = (int)Ms.NE;
+    const size_t NP = (size_t)Ms.NP;
+    const int e = eb * EPB + sub;
     const bool act = (nd < 125) && (e < NE);
 
     __shared__ float tabs[7 * 25];
@@ -127,16 +161,49 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
 
     const int l = nd % 5, m = (nd / 5) % 5, n = nd / 25;
     const int NX = Ms.NX, NY = Ms.NY;
-    const int ei = (int)(e % NX), ej = (int)((e / NX) % NY), ek = (int)(e / ((long long)NX * NY));
-    const long long p = e * 125 + nd;
+    const int ei = e % NX, ej = (e / NX) % NY, ek = e / (NX * NY);
+    const int p = e * 125 + nd;
+    // element in an absorbing layer? (uniform over the element's wavefronts)
+    const bool pml = act && (Ms.PMLX[ei] | Ms.PMLY[ej] | Ms.PMLZ[ek]);
+    const bool keep_d = (MODE != 0) || pml;
 
-    // gather v (SOURCE/ses3d_evolution.f90:27-51)
+    // All loads of the step are issued before the first barrier.
     float vx = 0.f, vy = 0.f, vz = 0.f;
+    float d9[9], s9[9], Mv[18];
+    float lam = 0.f, mu = 0.f, A = 0.f, B = 0.f, C = 0.f, QQ = 1.f;
     if (act) {
+        // gather v (SOURCE/ses3d_evolution.f90:27-51)
         const long long g = gidx(4 * ei + l, 4 * ej + m, 4 * ek + n, Ms.NBX, Ms.NBY);
         vx = E.v[g];
         vy = E.v[Ms.NGB + g];
         vz = E.v[2 * Ms.NGB + g];
+        if (keep_d) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) d9[c] = ldnt(E.d + c * NP + p);
+        }
+        s9[EXX] = ldnt(E.s + EXX * NP + p);
+        s9[EYY] = ldnt(E.s + EYY * NP + p);
+        s9[EZZ] = ldnt(E.s + EZZ * NP + p);
+        s9[EXY] = ldnt(E.s + EXY * NP + p);
+        s9[EXZ] = ldnt(E.s + EXZ * NP + p);
+        s9[EYZ] = ldnt(E.s + EYZ * NP + p);
+        if (pml) {
+            s9[EYX] = ldnt(E.s + EYX * NP + p);
+            s9[EZX] = ldnt(E.s + EZX * NP + p);
+            s9[EZY] = ldnt(E.s + EZY * NP + p);
+        }
+        if (DISS) {
+#pragma unroll
+            for (int c = 0; c < 18; ++c) Mv[c] = ldnt(E.M + c * NP + p);
+            QQ = ldnt(Ms.Q + p);
+        }
+        lam = ldnt(Ms.lam + p);
+        mu = ldnt(Ms.mu + p);
+        if (ANISO) {
+            A = ldnt(Ms.A + p);
+            B = ldnt(Ms.B + p);
+            C = ldnt(Ms.C + p);
+        }
     }
     sv[sub][0][nd] = vx;
     sv[sub][1][nd] = vy;
@@ -174,26 +241,26 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
         exy = exy / r;
         eyx = -((vy * ct) / r) + eyx / rst;
         // displacement gradients (:109-117)
-        float *d = E.d;
-        float dxux = d[EXX * NP + p], dyuy = d[EYY * NP + p], dzuz = d[EZZ * NP + p];
-        float dxuy = d[EXY * NP + p], dyux = d[EYX * NP + p], dxuz = d[EXZ * NP + p];
-        float dzux = d[EZX * NP + p], dyuz = d[EYZ * NP + p], dzuy = d[EZY * NP + p];
-        dzuz = dzuz + dt * ezz;
-        dyuy = dyuy + dt * eyy;
-        dxux = dxux + dt * exx;
-        dzux = dzux + dt * ezx;
-        dxuz = dxuz + dt * exz;
-        dzuy = dzuy + dt * ezy;
-        dyuz = dyuz + dt * eyz;
-        dxuy = dxuy + dt * exy;
-        dyux = dyux + dt * eyx;
-        d[EXX * NP + p] = dxux; d[EYY * NP + p] = dyuy; d[EZZ * NP + p] = dzuz;
-        d[EXY * NP + p] = dxuy; d[EYX * NP + p] = dyux; d[EXZ * NP + p] = dxuz;
-        d[EZX * NP + p] = dzux; d[EYZ * NP + p] = dyuz; d[EZY * NP + p] = dzuy;
-        // absorbing-layer terms (:123-134); exact no-op outside the layers
+        float dxux = 0.f, dyuy = 0.f, dzuz = 0.f, dxuy = 0.f, dyux = 0.f, dxuz = 0.f, dzux = 0.f, dyuz = 0.f, dzuy = 0.f;
+        if (keep_d) {
+            dzuz = d9[EZZ] + dt * ezz;
+            dyuy = d9[EYY] + dt * eyy;
+            dxux = d9[EXX] + dt * exx;
+            dzux = d9[EZX] + dt * ezx;
+            dxuz = d9[EXZ] + dt * exz;
+            dzuy = d9[EZY] + dt * ezy;
+            dyuz = d9[EYZ] + dt * eyz;
+            dxuy = d9[EXY] + dt * exy;
+            dyux = d9[EYX] + dt * eyx;
+            float *d = E.d;
+            stnt(d + EXX * NP + p, dxux); stnt(d + EYY * NP + p, dyuy); stnt(d + EZZ * NP + p, dzuz);
+            stnt(d + EXY * NP + p, dxuy); stnt(d + EYX * NP + p, dyux); stnt(d + EXZ * NP + p, dxuz);
+            stnt(d + EZX * NP + p, dzux); stnt(d + EYZ * NP + p, dyuz); stnt(d + EZY * NP + p, dzuy);
+        }
+        // absorbing-layer terms (:123-134); exact no-ops outside the layers
         float px = 0.f, py = 0.f, pz = 0.f, taper = 1.f;
-        const unsigned ix = Ms.IX[ei * 5 + l], iy = Ms.IY[ej * 5 + m], iz = Ms.IZ[ek * 5 + n];
-        if (ix | iy | iz) {
+        if (pml) {
+            const unsigned ix = Ms.IX[ei * 5 + l], iy = Ms.IY[ej * 5 + m], iz = Ms.IZ[ek * 5 + n];
             const float *lut = Ms.LUT + (((size_t)iz * Ms.KY + iy) * Ms.KX + ix) * 5;
             px = lut[0]; py = lut[1]; pz = lut[2]; taper = lut[4];
             exx = exx + (pz + py) * dxux;
@@ -210,31 +277,35 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
         exz = (exz + ezx) / 2.0f;
         eyz = (eyz + ezy) / 2.0f;
         // stress rates (:146-202)
-        const float mu = Ms.mu[p], A = Ms.A[p], B = Ms.B[p], C = Ms.C[p];
         float sxx, syy, szz, sxy, sxz, syz;
-        float Mv[18];
         if (DISS) {
-            const float QQ = Ms.Q[p];
             const float tau = 1.0f / QQ;
-            const float kappa = Ms.lam[p] + (2.0f * mu) / 3.0f;
+            const float kappa = lam + (2.0f * mu) / 3.0f;
             const float mut = (1.0f + Ms.sum_D_p * tau) * mu;
-#pragma unroll
-            for (int c = 0; c < 18; ++c) Mv[c] = E.M[c * NP + p];
             float Mdx = ((0.f + Mv[0]) + Mv[1]) + Mv[2];
             float Mdy = ((0.f + Mv[3]) + Mv[4]) + Mv[5];
             float Mdz = ((0.f + Mv[6]) + Mv[7]) + Mv[8];
             const float Md = (Mdx + Mdy) + Mdz;
             const float mt2 = (2.0f * mu) * tau;
             const float L = (kappa - (2.0f * mut) / 3.0f) * ((exx + eyy) + ezz) - (mt2 * Md) / 3.0f;
-            sxx = (((L + (2.0f * mut) * exx) + mt2 * Mdx) + C * ezz) + A * (eyy + exx);
-            syy = (((L + (2.0f * mut) * eyy) + mt2 * Mdy) + C * ezz) + A * (eyy + exx);
-            szz = ((L + (2.0f * mut) * ezz) + mt2 * Mdz) + C * (eyy + exx);
+            sxx = (L + (2.0f * mut) * exx) + mt2 * Mdx;
+            syy = (L + (2.0f * mut) * eyy) + mt2 * Mdy;
+            szz = (L + (2.0f * mut) * ezz) + mt2 * Mdz;
+            if (ANISO) {
+                sxx = (sxx + C * ezz) + A * (eyy + exx);
+                syy = (syy + C * ezz) + A * (eyy + exx);
+                szz = szz + C * (eyy + exx);
+            }
             Mdx = ((0.f + Mv[9]) + Mv[10]) + Mv[11];
             Mdy = ((0.f + Mv[12]) + Mv[13]) + Mv[14];
             Mdz = ((0.f + Mv[15]) + Mv[16]) + Mv[17];
             sxy = (2.0f * mut) * exy + mt2 * Mdx;
-            sxz = ((2.0f * mut) * exz + mt2 * Mdy) + (2.0f * B) * exz;
-            syz = ((2.0f * mut) * eyz + mt2 * Mdz) + (2.0f * B) * eyz;
+            sxz = (2.0f * mut) * exz + mt2 * Mdy;
+            syz = (2.0f * mut) * eyz + mt2 * Mdz;
+            if (ANISO) {
+                sxz = sxz + (2.0f * B) * exz;
+                syz = syz + (2.0f * B) * eyz;
+            }
             // memory variables (:334-360), from the post-absorbing, symmetrised rates
             const float ev6[6] = {exx, eyy, ezz, exy, exz, eyz};
 #pragma unroll
@@ -244,50 +315,78 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
                     const float Mo = Mv[c * 3 + k];
                     const float Mn = (Mo - (dt * Mo) / Ms.tau_p[k]) - Ms.Lk[k] * ev6[c];
                     Mv[c * 3 + k] = Mn;
-                    E.M[(c * 3 + k) * NP + p] = Mn;
+                    stnt(E.M + (c * 3 + k) * NP + p, Mn);
                 }
         } else {
-            const float L = Ms.lam[p] * ((exx + eyy) + ezz);
-            sxx = ((L + (2.0f * mu) * exx) + C * ezz) + A * (eyy + exx);
-            syy = ((L + (2.0f * mu) * eyy) + C * ezz) + A * (eyy + exx);
-            szz = (L + (2.0f * mu) * ezz) + C * (eyy + exx);
+            const float L = lam * ((exx + eyy) + ezz);
+            sxx = L + (2.0f * mu) * exx;
+            syy = L + (2.0f * mu) * eyy;
+            szz = L + (2.0f * mu) * ezz;
+            if (ANISO) {
+                sxx = (sxx + C * ezz) + A * (eyy + exx);
+                syy = (syy + C * ezz) + A * (eyy + exx);
+                szz = szz + C * (eyy + exx);
+            }
             sxy = (2.0f * mu) * exy;
-            sxz = (2.0f * mu) * exz + (2.0f * B) * exz;
-            syz = (2.0f * mu) * eyz + (2.0f * B) * eyz;
+            sxz = (2.0f * mu) * exz;
+            syz = (2.0f * mu) * eyz;
+            if (ANISO) {
+                sxz = sxz + (2.0f * B) * exz;
+                syz = syz + (2.0f * B) * eyz;
+            }
         }
         // split-stress integration (:208-219)
         float *s = E.s;
-        float s9[9];
-        s9[EXX] = s[EXX * NP + p]; s9[EYY] = s[EYY * NP + p]; s9[EZZ] = s[EZZ * NP + p];
-        s9[EXY] = s[EXY * NP + p]; s9[EYX] = s[EYX * NP + p]; s9[EXZ] = s[EXZ * NP + p];
-        s9[EZX] = s[EZX * NP + p]; s9[EYZ] = s[EYZ * NP + p]; s9[EZY] = s[EZY * NP + p];
-        s9[EXX] = s9[EXX] + dt * (sxx - px * s9[EXX]);
-        s9[EYY] = s9[EYY] + dt * (syy - py * s9[EYY]);
-        s9[EZZ] = s9[EZZ] + dt * (szz - pz * s9[EZZ]);
-        s9[EXY] = s9[EXY] + dt * (sxy - px * s9[EXY]);
-        s9[EYX] = s9[EYX] + dt * (sxy - py * s9[EYX]);
-        s9[EXZ] = s9[EXZ] + dt * (sxz - px * s9[EXZ]);
-        s9[EZX] = s9[EZX] + dt * (sxz - pz * s9[EZX]);
-        s9[EYZ] = s9[EYZ] + dt * (syz - py * s9[EYZ]);
-        s9[EZY] = s9[EZY] + dt * (syz - pz * s9[EZY]);
-        // moment-tensor stress glut (misc.f90:131-142, added at :251-264)
+        if (pml) {
+            s9[EXX] = s9[EXX] + dt * (sxx - px * s9[EXX]);
+            s9[EYY] = s9[EYY] + dt * (syy - py * s9[EYY]);
+            s9[EZZ] = s9[EZZ] + dt * (szz - pz * s9[EZZ]);
+            s9[EXY] = s9[EXY] + dt * (sxy - px * s9[EXY]);
+            s9[EYX] = s9[EYX] + dt * (sxy - py * s9[EYX]);
+            s9[EXZ] = s9[EXZ] + dt * (sxz - px * s9[EXZ]);
+            s9[EZX] = s9[EZX] + dt * (sxz - pz * s9[EZX]);
+            s9[EYZ] = s9[EYZ] + dt * (syz - py * s9[EYZ]);
+            s9[EZY] = s9[EZY] + dt * (syz - pz * s9[EZY]);
+        } else {
+            s9[EXX] = s9[EXX] + dt * sxx;
+            s9[EYY] = s9[EYY] + dt * syy;
+            s9[EZZ] = s9[EZZ] + dt * szz;
+            s9[EXY] = s9[EXY] + dt * sxy;
+            s9[EXZ] = s9[EXZ] + dt * sxz;
+            s9[EYZ] = s9[EYZ] + dt * syz;
+        }
+        // moment-tensor stress glut (misc.f90:131-142, added at :251-264); the
+        // paired glut components are equal, so the stored pair stays equal
+        const int ncomp = pml ? 9 : 6;
+        const int slot6[9] = {EXX, EYY, EZZ, EXY, EXZ, EYZ, EYX, EZX, EZY};
         if (e == E.src_elem) {
             const float so = E.so[it - 1];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) ss[sub][c][nd] = s9[c] + (0.0f - E.src_md[c * 125 + nd] * so);
+            for (int k = 0; k < 9; ++k)
+                if (k < ncomp) {
+                    const int c = slot6[k];
+                    ss[sub][c][nd] = s9[c] + (0.0f - E.src_md[c * 125 + nd] * so);
+                }
         } else {
 #pragma unroll
-            for (int c = 0; c < 9; ++c) ss[sub][c][nd] = s9[c];
+            for (int k = 0; k < 9; ++k)
+                if (k < ncomp) ss[sub][slot6[k]][nd] = s9[slot6[k]];
         }
-        // stress taper (:368-371); the divergence uses the untapered stresses
+        // stress taper (:368-371), 1 outside the layers; the divergence uses
+        // the untapered stresses
+        if (pml) {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) s[c * NP + p] = s9[c] * taper;
+            for (int c = 0; c < 9; ++c) stnt(s + c * NP + p, s9[c] * taper);
+        } else {
+            stnt(s + EXX * NP + p, s9[EXX]); stnt(s + EYY * NP + p, s9[EYY]); stnt(s + EZZ * NP + p, s9[EZZ]);
+            stnt(s + EXY * NP + p, s9[EXY]); stnt(s + EXZ * NP + p, s9[EXZ]); stnt(s + EYZ * NP + p, s9[EYZ]);
+        }
 
         if (MODE == 1) {
             // forward-field store (ses3d_main.f90:237-247)
             const int sl = E.slot[it];
             if (sl >= 0) {
-                float *h = E.hist + (long long)sl * 9 * NP;
+                float *h = E.hist + (size_t)sl * 9 * NP;
                 h[0 * NP + p] = vx; h[1 * NP + p] = vy; h[2 * NP + p] = vz;
                 h[3 * NP + p] = dxux; h[4 * NP + p] = dyuy; h[5 * NP + p] = dzuz;
                 h[6 * NP + p] = (dxuy + dyux) / 2.0f;
@@ -300,7 +399,7 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
             // increments in the reference's order, accumulated in float64
             const int sl = E.slot[it];
             if (sl >= 0) {
-                const float *h = E.hist + (long long)sl * 9 * NP;
+                const float *h = E.hist + (size_t)sl * 9 * NP;
                 const float vxf = h[p], vyf = h[NP + p], vzf = h[2 * NP + p];
                 const float exxf = h[3 * NP + p], eyyf = h[4 * NP + p], ezzf = h[5 * NP + p];
                 const float exyf = h[6 * NP + p], exzf = h[7 * NP + p], eyzf = h[8 * NP + p];
@@ -308,7 +407,7 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
                 const float tra = (dxux + dyuy) + dzuz, trf = (exxf + eyyf) + ezzf;
                 const float ede = ((((dxux * exxf + dyuy * eyyf) + dzuz * ezzf) + (2.0f * exya) * exyf) +
                                    (2.0f * exza) * exzf) + (2.0f * eyza) * eyzf;
-                const float rhoinv = Ms.rhoinv[p], lam = Ms.lam[p];
+                const float rhoinv = Ms.rhoinv[p];
                 const float rho = rhoinv > 0.f ? 1.0f / rhoinv : 0.f;
                 const float cs = sqrtf(mu * rhoinv);
                 const float cp = sqrtf((lam + 2.0f * mu) * rhoinv);
@@ -342,7 +441,9 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
     __syncthreads();
 
     if (act) {
-        // weak-form stress divergence (:237-270)
+        // weak-form stress divergence (:237-270); outside the layers the second
+        // member of each stress pair is read from the first
+        const int cyx = pml ? EYX : EXY, czx = pml ? EZX : EXZ, czy = pml ? EZY : EYZ;
         const float zc = Ms.Z[ek * 5 + n], sa = Ms.S[ei * 5 + l];
         float ax = 0.f, ay = 0.f, az = 0.f;
 #pragma unroll
@@ -352,13 +453,13 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
             const float zq = Ms.Z[ek * 5 + q];
             const float wz = (((WDL[n * 5 + q] * zq) * zq) * sa) * DUM3[l * 5 + m];
             const int pz_ = l + 5 * (m + 5 * q), py_ = l + 5 * (q + 5 * n), px_ = q + 5 * (m + 5 * n);
-            ax = ((ax + ss[sub][EZX][pz_] * wz) + ss[sub][EYX][py_] * wy) + ss[sub][EXX][px_] * wx;
-            ay = ((ay + ss[sub][EZY][pz_] * wz) + ss[sub][EYY][py_] * wy) + ss[sub][EXY][px_] * wx;
+            ax = ((ax + ss[sub][czx][pz_] * wz) + ss[sub][cyx][py_] * wy) + ss[sub][EXX][px_] * wx;
+            ay = ((ay + ss[sub][czy][pz_] * wz) + ss[sub][EYY][py_] * wy) + ss[sub][EXY][px_] * wx;
             az = ((az + ss[sub][EZZ][pz_] * wz) + ss[sub][EYZ][py_] * wy) + ss[sub][EXZ][px_] * wx;
         }
-        E.f[p] = ax;
-        E.f[NP + p] = ay;
-        E.f[2 * NP + p] = az;
+        stnt(E.f + p, ax);
+        stnt(E.f + NP + p, ay);
+        stnt(E.f + 2 * NP + p, az);
     }
 }
 
@@ -438,16 +539,38 @@ __global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict
         if (bz > 0) { ze[nzs] = bz - 1; zl[nzs] = 4; ++nzs; }
         if (bz < NZ) { ze[nzs] = bz; zl[nzs] = 0; ++nzs; }
     }
-    float Sx = 0.f, Sy = 0.f, Sz = 0.f;
-    for (int ii = 0; ii < nxs; ++ii)
-        for (int jj = 0; jj < nys; ++jj)
-            for (int kk = 0; kk < nzs; ++kk) {
-                const long long q = ((long long)xe[ii] + (long long)NX * (ye[jj] + (long long)NY * ze[kk])) * 125 +
-                                    (xl[ii] + 5 * (yl[jj] + 5 * zl[kk]));
-                Sx = Sx + E.f[q];
-                Sy = Sy + E.f[NP + q];
-                Sz = Sz + E.f[2 * NP + q];
+    // load the <=8 contributions first (independent loads), then add them in
+    // the reference's order: sharers sorted by (i, j, k), i slowest
+    float fx[8], fy[8], fz[8];
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                const int sl = ii * 4 + jj * 2 + kk;
+                fx[sl] = fy[sl] = fz[sl] = 0.f;
+                if (ii < nxs && jj < nys && kk < nzs) {
+                    const long long q = ((long long)xe[ii] + (long long)NX * (ye[jj] + (long long)NY * ze[kk])) * 125 +
+                                        (xl[ii] + 5 * (yl[jj] + 5 * zl[kk]));
+                    fx[sl] = ldnt(E.f + q);
+                    fy[sl] = ldnt(E.f + NP + q);
+                    fz[sl] = ldnt(E.f + 2 * NP + q);
+                }
             }
+    float Sx = 0.f, Sy = 0.f, Sz = 0.f;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk)
+                if (ii < nxs && jj < nys && kk < nzs) {
+                    const int sl = ii * 4 + jj * 2 + kk;
+                    Sx = Sx + fx[sl];
+                    Sy = Sy + fy[sl];
+                    Sz = Sz + fz[sl];
+                }
     // absorbing profile and taper of the global node: the reference copies the
     // element-local value of the last element in loop order (init.f90:690-709)
     float pg = 0.f, tg = 1.f;
@@ -591,6 +714,9 @@ struct ses3d_hip_context {
           *rhoinv = nullptr, *MM = nullptr;
     float *Z = nullptr, *S = nullptr, *COT = nullptr, *LUT = nullptr, *w5 = nullptr;
     unsigned short *IX = nullptr, *IY = nullptr, *IZ = nullptr;
+    unsigned char *PMLX = nullptr, *PMLY = nullptr, *PMLZ = nullptr;
+    std::vector<unsigned char> pmlx, pmly, pmlz;   // host copies
+    bool aniso = true;                             // any of A, B, C nonzero
     MeshDev mesh_h{};
     MeshDev *mesh_d = nullptr;
     std::vector<EventHost> evh;
@@ -681,12 +807,15 @@ long long node_blocks(const ses3d_hip_context *c) { return (long long)c->NBX * c
 int launch_element(ses3d_hip_context *c, int it, int mode) {
     const int bpe = elem_blocks(c);
     dim3 grid((unsigned)(bpe * c->n_events)), block(ELEM_THREADS);
-    const bool diss = c->cfg.is_diss != 0;
-#define LAUNCH_E(D, MO) hipLaunchKernelGGL((k_element<D, MO>), grid, block, 0, c->stream, c->mesh_d, c->ev_d, it, bpe)
-    if (diss) {
-        if (mode == 0) LAUNCH_E(1, 0); else if (mode == 1) LAUNCH_E(1, 1); else LAUNCH_E(1, 2);
-    } else {
-        if (mode == 0) LAUNCH_E(0, 0); else if (mode == 1) LAUNCH_E(0, 1); else LAUNCH_E(0, 2);
+    const int key = (c->cfg.is_diss ? 100 : 0) + mode * 10 + (c->aniso ? 1 : 0);
+#define LAUNCH_E(D, MO, AN)                                                                                   \
+    case D * 100 + MO * 10 + AN:                                                                              \
+        hipLaunchKernelGGL((k_element<D, MO, AN>), grid, block, 0, c->stream, c->mesh_d, c->ev_d, it, bpe); \
+        break;
+    switch (key) {
+        LAUNCH_E(0, 0, 0) LAUNCH_E(0, 0, 1) LAUNCH_E(0, 1, 0) LAUNCH_E(0, 1, 1) LAUNCH_E(0, 2, 0) LAUNCH_E(0, 2, 1)
+        LAUNCH_E(1, 0, 0) LAUNCH_E(1, 0, 1) LAUNCH_E(1, 1, 0) LAUNCH_E(1, 1, 1) LAUNCH_E(1, 2, 0) LAUNCH_E(1, 2, 1)
+        default: return fail(SES3D_HIP_EINVAL, "no element kernel for this mode");
     }
 #undef LAUNCH_E
     HIP_TRY(hipGetLastError());
@@ -804,6 +933,10 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
     if (cfg->pml < 0 || cfg->pml > cfg->nx + 1 || cfg->pml > cfg->ny + 1 || cfg->pml > cfg->nz + 1)
         return fail(SES3D_HIP_EINVAL, "pml width exceeds the mesh");
     if (cfg->adjoint_flag < 0 || cfg->adjoint_flag > 2) return fail(SES3D_HIP_EINVAL, "adjoint_flag must be 0, 1 or 2");
+    {
+        const long long ne = (long long)(cfg->nx + 1) * (cfg->ny + 1) * (cfg->nz + 1);
+        if (ne * 125 >= (1LL << 31)) return fail(SES3D_HIP_EINVAL, "more than 2^31 GLL points in one section");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(SES3D_HIP_EDEVICE, "no HIP device available");
@@ -815,7 +948,7 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
     c->NE = (long long)c->NX * c->NY * c->NZ;
     c->NP = c->NE * 125;
     c->NGB = (long long)c->NBX * c->NBY * c->NBZ * 64;
-    hipGetDevice(&c->device);
+    (void)hipGetDevice(&c->device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(SES3D_HIP_EDEVICE, "hipStreamCreate failed");
@@ -857,10 +990,10 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
 
 void ses3d_hip_destroy(ses3d_hip_context *c) {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (void *p : c->allocs) hipFree(p);
-    for (auto &e : c->tev) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    for (auto &e : c->tev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -876,6 +1009,9 @@ int ses3d_hip_set_model(ses3d_hip_context *c, const float *rhoinv, const float *
         ref_to_em(c, src[f], tmp.data());
         TRY(upload(c, dst[f], tmp.data(), sizeof(float) * c->NP));
     }
+    c->aniso = false;
+    for (long long i = 0; i < c->NP && !c->aniso; ++i)
+        if (A[i] != 0.0f || B[i] != 0.0f || C[i] != 0.0f) c->aniso = true;
     c->model_set = true;
     c->initialised = false;
     return 0;
@@ -930,7 +1066,21 @@ int ses3d_hip_init(ses3d_hip_context *c) {
         TRY(dalloc(c, &c->IZ, ms.IZ.size()));
         TRY(dalloc(c, &c->LUT, ms.LUT.size()));
         TRY(dalloc(c, &c->w5, 5));
+        TRY(dalloc(c, &c->PMLX, c->NX));
+        TRY(dalloc(c, &c->PMLY, c->NY));
+        TRY(dalloc(c, &c->PMLZ, c->NZ));
     }
+    auto any_layer = [](const std::vector<uint16_t> &ids, int ne, std::vector<unsigned char> &out) {
+        out.assign(ne, 0);
+        for (int e = 0; e < ne; ++e)
+            for (int q = 0; q < NN; ++q) out[e] |= ids[e * NN + q] != 0;
+    };
+    any_layer(ms.IX, c->NX, c->pmlx);
+    any_layer(ms.IY, c->NY, c->pmly);
+    any_layer(ms.IZ, c->NZ, c->pmlz);
+    TRY(upload(c, c->PMLX, c->pmlx.data(), c->NX));
+    TRY(upload(c, c->PMLY, c->pmly.data(), c->NY));
+    TRY(upload(c, c->PMLZ, c->pmlz.data(), c->NZ));
     TRY(upload(c, c->Z, ms.z.data(), sizeof(float) * ms.z.size()));
     TRY(upload(c, c->S, ms.sin_x.data(), sizeof(float) * ms.sin_x.size()));
     TRY(upload(c, c->COT, ms.cot_x.data(), sizeof(float) * ms.cot_x.size()));
@@ -960,6 +1110,7 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     M.lam = c->lam; M.mu = c->mu; M.A = c->A; M.B = c->B; M.C = c->C; M.Q = c->Q; M.rhoinv = c->rhoinv;
     M.MM = c->MM; M.Z = c->Z; M.S = c->S; M.COT = c->COT; M.IX = c->IX; M.IY = c->IY; M.IZ = c->IZ;
     M.LUT = c->LUT;
+    M.PMLX = c->PMLX; M.PMLY = c->PMLY; M.PMLZ = c->PMLZ;
     TRY(upload(c, c->mesh_d, &M, sizeof(MeshDev)));
     // mass matrix
     {
@@ -1112,8 +1263,8 @@ int ses3d_hip_run(ses3d_hip_context *c, int32_t it_begin, int32_t n_steps) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
         c->t_total = ms;
-        hipEventDestroy(t0);
-        hipEventDestroy(t1);
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
     }
     return 0;
 }
@@ -1160,10 +1311,31 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
     static const char *sn[9] = {"sxx", "syy", "szz", "sxy", "syx", "sxz", "szx", "syz", "szy"};
     static const char *mn[6] = {"Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz"};
     std::vector<float> tmp;
+    // outside the absorbing layers the second member of each split-stress pair
+    // is the first (only the first is stored there)
+    static const int pair[9] = {EXX, EYY, EZZ, EXY, EXY, EXZ, EXZ, EYZ, EYZ};
     for (int k = 0; k < 9; ++k) {
-        if (nm == dn[k] || nm == sn[k]) {
+        if (nm == dn[k]) {
+            if (c->cfg.adjoint_flag == 0)
+                return fail(SES3D_HIP_EINVAL, "with adjoint_flag=0 the displacement gradient is integrated only "
+                                              "inside the absorbing layers (the only place it acts); use adjoint_flag 1 or 2");
             tmp.resize(NP);
-            TRY(download(c, tmp.data(), (nm == dn[k] ? E.d : E.s) + k * NP, sizeof(float) * NP));
+            TRY(download(c, tmp.data(), E.d + k * NP, sizeof(float) * NP));
+            em_to_ref(c, tmp.data(), out);
+            return 0;
+        }
+        if (nm == sn[k]) {
+            tmp.resize(NP);
+            TRY(download(c, tmp.data(), E.s + k * NP, sizeof(float) * NP));
+            if (pair[k] != k) {
+                std::vector<float> first(NP);
+                TRY(download(c, first.data(), E.s + pair[k] * NP, sizeof(float) * NP));
+                for (long long e = 0; e < c->NE; ++e) {
+                    const int i = (int)(e % c->NX), j = (int)((e / c->NX) % c->NY), kk = (int)(e / ((long long)c->NX * c->NY));
+                    if (!(c->pmlx[i] | c->pmly[j] | c->pmlz[kk]))
+                        std::memcpy(&tmp[e * 125], &first[e * 125], sizeof(float) * 125);
+                }
+            }
             em_to_ref(c, tmp.data(), out);
             return 0;
         }

@@ -15,7 +15,7 @@ from typing import Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libses3d_hip.so")
+LIB_PATH = os.environ.get("SES3D_HIP_LIB", os.path.join(HERE, "libses3d_hip.so"))
 
 _FIELDS_LOCAL = ("vx", "vy", "vz", "dxux", "dyuy", "dzuz", "dxuy", "dyux", "dxuz", "dzux", "dyuz",
                  "dzuy", "sxx", "syy", "szz", "sxy", "syx", "sxz", "szx", "syz", "szy")

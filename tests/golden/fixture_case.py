@@ -18,6 +18,7 @@ CASES = {
     "elastic_mt": ((5, 6, 3), 60, 0.2, 0, 10, (50.2, 20.7, 25000.0), 3, False),
     "anelastic_adjoint": ((5, 5, 4), 50, 0.2, 1, 10, (49.3, 19.1, 60000.0), 4, True),
     "force_r": ((4, 5, 3), 40, 0.2, 0, 3, (51.0, 21.5, 15000.0), 3, False),
+    "aniso_mt": ((6, 5, 4), 40, 0.2, 1, 10, (50.5, 19.7, 40000.0), 3, False),
 }
 
 
@@ -39,6 +40,11 @@ def compact_inputs(name: str, seed: int = 11) -> dict:
         "lambda_z": (rho * vp * vp - 2.0 * mu).astype(np.float32),
         "Q_z": M.q_mu_by_depth(z).astype(np.float32),
     }
+    if name.startswith("aniso"):
+        # depth-dependent anisotropy parameters of a few percent of mu
+        out["A_z"] = (0.04 * mu * np.cos(z / 9e4)).astype(np.float32)
+        out["B_z"] = (-0.03 * mu * np.sin(z / 7e4)).astype(np.float32)
+        out["C_z"] = (0.05 * mu * np.cos(z / 5e4)).astype(np.float32)
     rec = M.surface_receiver_grid(setup, 2, 2)
     out["rec"] = np.stack([rec.colat, rec.lon, rec.depth], 1)
     return out
@@ -59,8 +65,9 @@ def write_tree(fx: dict, root: str, adjoint_flag: int = None) -> None:
         return np.asfortranarray(np.broadcast_to(np.asarray(v, np.float32)[None, None, :, None, None, :], shape))
 
     model = {"rhoinv": bz(fx["rhoinv_z"]), "mu": bz(fx["mu_z"]), "lambda": bz(fx["lambda_z"]),
-             "A": np.zeros(shape, np.float32, order="F"), "B": np.zeros(shape, np.float32, order="F"),
-             "C": np.zeros(shape, np.float32, order="F")}
+             "A": bz(fx["A_z"]) if "A_z" in fx else np.zeros(shape, np.float32, order="F"),
+             "B": bz(fx["B_z"]) if "B_z" in fx else np.zeros(shape, np.float32, order="F"),
+             "C": bz(fx["C_z"]) if "C_z" in fx else np.zeros(shape, np.float32, order="F")}
     if is_diss:
         model["Q"] = bz(fx["Q_z"])
     src = [float(v) for v in fx["src"]]

@@ -9,8 +9,8 @@ from tests.cases import make_case, oracle_for
 
 pytestmark = pytest.mark.gpu
 
-FIELDS = ("dxux", "dyuy", "dzuz", "dxuy", "dyux", "dxuz", "dzux", "dyuz", "dzuy",
-          "sxx", "syy", "szz", "sxy", "syx", "sxz", "szx", "syz", "szy")
+DGRAD = ("dxux", "dyuy", "dzuz", "dxuy", "dyux", "dxuz", "dzux", "dyuz", "dzuy")
+STRESS = ("sxx", "syy", "szz", "sxy", "syx", "sxz", "szx", "syz", "szy")
 
 
 def _gpu(run, **kw):
@@ -35,13 +35,15 @@ def _assert_same(a, b, what):
                              f"{np.abs(a - b).max():.3e}, max|ref| {np.abs(b).max():.3e}")
 
 
-@pytest.mark.parametrize("is_diss", [0, 1])
-def test_forward_bitexact(tmp_path, is_diss):
+@pytest.mark.parametrize("is_diss,mode", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_forward_bitexact(tmp_path, is_diss, mode):
+    """mode 0: forward only; mode 1: forward storing the forward field (the
+    displacement gradient is then maintained everywhere and compared too)."""
     run = make_case(str(tmp_path), nel=(6, 7, 4), nt=40, is_diss=bool(is_diss), seed=3 + is_diss)
     ev = run.events[0]
-    s = _gpu(run)
-    o = oracle_for(run, ev)
-    print(f"case elastic={not is_diss} nel=6x7x4 nt=40", flush=True)
+    s = _gpu(run, adjoint_flag=mode)
+    o = oracle_for(run, ev, adjoint_flag=mode)
+    print(f"case elastic={not is_diss} mode={mode} nel=6x7x4 nt=40", flush=True)
     s.run(1, 20)
     o.run(1, 20)
     _assert_same(s.field("MM_global"), o.get("MM_global"), "MM_global")
@@ -50,7 +52,7 @@ def test_forward_bitexact(tmp_path, is_diss):
     o.run(21, 20)
     for c in ("vx", "vy", "vz"):
         _assert_same(s.field(c + "_global"), o.get(c + "g"), c + "_global@40")
-    for f in FIELDS:
+    for f in STRESS + (DGRAD if mode else ()):
         _assert_same(s.field(f), o.get(f), f)
     if is_diss:
         for f in ("Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz"):
@@ -74,3 +76,29 @@ def test_point_force_bitexact(tmp_path, source_type):
     for c in ("vx", "vy", "vz"):
         _assert_same(s.field(c + "_global"), o.get(c + "g"), c)
     _assert_same(s.seismograms(), o.get("seismograms"), "seismograms")
+
+
+def test_anisotropic_bitexact(tmp_path):
+    """Nonzero A, B, C (the anisotropic terms of ses3d_evolution.f90:163-200)."""
+    run = make_case(str(tmp_path), nel=(6, 5, 4), nt=30, is_diss=True, seed=9)
+    rng = np.random.default_rng(1)
+    for k in ("A", "B", "C"):
+        run.model[k] = np.asfortranarray((run.model["mu"] * rng.uniform(-0.05, 0.05, run.model["mu"].shape)).astype(np.float32))
+    ev = run.events[0]
+    s = _gpu(run, adjoint_flag=1)
+    o = oracle_for(run, ev, adjoint_flag=1)
+    s.run()
+    o.run(1, ev.nt)
+    for c in ("vx", "vy", "vz"):
+        _assert_same(s.field(c + "_global"), o.get(c + "g"), c)
+    for f in STRESS + DGRAD:
+        _assert_same(s.field(f), o.get(f), f)
+    _assert_same(s.seismograms(), o.get("seismograms"), "seismograms")
+
+
+def test_forward_only_rejects_dgrad(tmp_path):
+    from ses3d_amd.solver import Ses3dError
+    run = make_case(str(tmp_path), nel=(5, 5, 4), nt=5)
+    s = _gpu(run, adjoint_flag=0)
+    with pytest.raises(Ses3dError):
+        s.field("dxux")

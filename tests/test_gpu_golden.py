@@ -24,7 +24,7 @@ def _solver(run, af=None):
     return s
 
 
-@pytest.mark.parametrize("name", ["elastic_mt", "force_r", "anelastic_adjoint"])
+@pytest.mark.parametrize("name", ["elastic_mt", "force_r", "anelastic_adjoint", "aniso_mt"])
 def test_gpu_matches_reference_fixture(name, tmp_path):
     fx = dict(np.load(os.path.join(GOLDEN, f"{name}.npz")))
     FC.write_tree(fx, str(tmp_path))

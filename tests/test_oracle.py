@@ -30,7 +30,7 @@ def _oracle(run, e, af, precision=32):
     return oracle_for(run, e, adjoint_flag=af, precision=precision)
 
 
-@pytest.mark.parametrize("name", ["elastic_mt", "force_r", "anelastic_adjoint"])
+@pytest.mark.parametrize("name", ["elastic_mt", "force_r", "anelastic_adjoint", "aniso_mt"])
 def test_oracle_matches_reference_fixture(name, oracle_lib, tmp_path):
     fx = _load(name)
     FC.write_tree(fx, str(tmp_path))
