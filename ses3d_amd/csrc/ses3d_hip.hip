@@ -509,19 +509,20 @@ __global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict
                                                        const EventDev *__restrict__ evs,
                                                        int nodeblocks_per_event, int n_events) {
     const MeshDev &Ms = *mesh;
-    const long long gt = (long long)xcd_remap(blockIdx.x, gridDim.x) * NODE_THREADS + threadIdx.x;
-    const long long nbt = gt >> 6;
-    const int ev = (int)(nbt / nodeblocks_per_event);
+    // 32-bit index math: n_events * padded nodes stays below 2^31 (checked on the host)
+    const int gt = xcd_remap(blockIdx.x, gridDim.x) * NODE_THREADS + threadIdx.x;
+    const int nbt = gt >> 6;
+    const int ev = nbt / nodeblocks_per_event;
     if (ev >= n_events) return;
-    const long long nb = nbt - (long long)ev * nodeblocks_per_event;
-    const int loc = (int)(gt & 63);
+    const int nb = nbt - ev * nodeblocks_per_event;
+    const int loc = gt & 63;
     const int NX = Ms.NX, NY = Ms.NY, NZ = Ms.NZ, NBX = Ms.NBX, NBY = Ms.NBY;
-    const int bx = (int)(nb % NBX), by = (int)((nb / NBX) % NBY), bz = (int)(nb / ((long long)NBX * NBY));
+    const int byz = nb / NBX;
+    const int bx = nb - byz * NBX, bz = byz / NBY, by = byz - bz * NBY;
     const int a = loc & 3, b = (loc >> 2) & 3, c = loc >> 4;
-    if (bz >= Ms.NBZ) return;
     if ((bx == NX && a) || (by == NY && b) || (bz == NZ && c)) return;
     const EventDev &E = evs[ev];
-    const long long NP = Ms.NP;
+    const size_t NP = (size_t)Ms.NP;
     // sharers per axis, ascending element index
     int xe[2], xl[2], nxs = 0, ye[2], yl[2], nys = 0, ze[2], zl[2], nzs = 0;
     if (a) { xe[0] = bx; xl[0] = a; nxs = 1; }
@@ -551,8 +552,7 @@ __global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict
                 const int sl = ii * 4 + jj * 2 + kk;
                 fx[sl] = fy[sl] = fz[sl] = 0.f;
                 if (ii < nxs && jj < nys && kk < nzs) {
-                    const long long q = ((long long)xe[ii] + (long long)NX * (ye[jj] + (long long)NY * ze[kk])) * 125 +
-                                        (xl[ii] + 5 * (yl[jj] + 5 * zl[kk]));
+                    const int q = (xe[ii] + NX * (ye[jj] + NY * ze[kk])) * 125 + (xl[ii] + 5 * (yl[jj] + 5 * zl[kk]));
                     fx[sl] = ldnt(E.f + q);
                     fy[sl] = ldnt(E.f + NP + q);
                     fz[sl] = ldnt(E.f + 2 * NP + q);
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict
             pg = lut[3]; tg = lut[4];
         }
     }
-    const long long g = (nb << 6) + loc;
+    const int g = (nb << 6) + loc;
     const float mm = Ms.MM[g];
     const float dt = Ms.dt;
     float *v = E.v;
@@ -692,12 +692,16 @@ struct EventHost {
     float *rec_w = nullptr;    // nr x 125
     float *force_w = nullptr, *force_stf = nullptr;
     double *grad = nullptr;
-    float *hist = nullptr;
+    float *hist = nullptr;       // sampled forward fields the kernels read/write (active buffer)
+    float *hist_own = nullptr;   // full history of every sampled step (allocated on first use)
     int n_slots = 0;
-    std::vector<int> slot_h;   // per forward step: history slot (adjoint_flag 1/2 store)
+    std::vector<int> slot_h;     // per forward step: index among the sampled steps, or -1
+    std::vector<int> slot_dev;   // the table currently on the device ([it] -> slot)
     int *slot_d = nullptr;
-    // checkpoints (forward_checkpointed)
+    // restart checkpoints (forward_checkpointed) and the segment buffer replayed into
     std::vector<float *> ckpt;
+    float *seg = nullptr;
+    int seg_slots = 0;
 };
 
 struct ses3d_hip_context {
@@ -728,6 +732,8 @@ struct ses3d_hip_context {
     ForceDev *forces_d = nullptr;
     int n_groups = 0;
     std::vector<int> saving_vector;   // 1..nt
+    int ckpt_k = 0;
+    bool ckpt_ready = false;
     std::vector<void *> allocs;
     int64_t bytes = 0;
     // timing
@@ -882,6 +888,8 @@ int collect_timing(ses3d_hip_context *c) {
     return 0;
 }
 
+int refresh_events(ses3d_hip_context *c);
+
 int build_slots(ses3d_hip_context *c, EventHost &E, bool allocate_history) {
     const int nt = c->cfg.nt, sa = c->cfg.samp_ad > 0 ? c->cfg.samp_ad : 1;
     E.slot_h.assign(nt + 1, -1);
@@ -890,7 +898,35 @@ int build_slots(ses3d_hip_context *c, EventHost &E, bool allocate_history) {
         if ((it % sa) == 0 || it == 1 || it == nt) E.slot_h[it] = n++;
     E.n_slots = n;
     if (!E.slot_d) TRY(dalloc(c, &E.slot_d, nt + 1));
-    if (allocate_history && !E.hist) TRY(dalloc(c, &E.hist, (size_t)n * 9 * c->NP));
+    if (allocate_history && !E.hist_own) TRY(dalloc(c, &E.hist_own, (size_t)n * 9 * c->NP));
+    if (allocate_history) E.hist = E.hist_own;
+    return 0;
+}
+
+// Device slot table of one event: fwd tables map it -> slot of step it,
+// adjoint tables map it -> slot of the paired forward step nt+1-it.
+int upload_slots(ses3d_hip_context *c, EventHost &E, const std::vector<int> &tab) {
+    E.slot_dev = tab;
+    HIP_TRY(hipMemcpyAsync(E.slot_d, E.slot_dev.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// full history of all sampled steps for the stored-history (disk-equivalent) path
+int ensure_full_history(ses3d_hip_context *c) {
+    const int nt = c->cfg.nt, af = c->cfg.adjoint_flag;
+    bool changed = false;
+    for (auto &E : c->evh) {
+        if (!E.hist_own) {
+            TRY(build_slots(c, E, true));
+            changed = true;
+        }
+        if (E.hist != E.hist_own) { E.hist = E.hist_own; changed = true; }
+        std::vector<int> sl(nt + 1, -1);
+        for (int it = 1; it <= nt; ++it) sl[it] = af == 1 ? E.slot_h[it] : E.slot_h[nt + 1 - it];
+        if (sl != E.slot_dev) TRY(upload_slots(c, E, sl));
+    }
+    if (changed) TRY(refresh_events(c));
     return 0;
 }
 
@@ -936,6 +972,9 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
     {
         const long long ne = (long long)(cfg->nx + 1) * (cfg->ny + 1) * (cfg->nz + 1);
         if (ne * 125 >= (1LL << 31)) return fail(SES3D_HIP_EINVAL, "more than 2^31 GLL points in one section");
+        const long long ngb = (long long)(cfg->nx + 2) * (cfg->ny + 2) * (cfg->nz + 2) * 64;
+        if (ngb * n_events >= (1LL << 31))
+            return fail(SES3D_HIP_EINVAL, "events x global nodes exceed 2^31: split the events over contexts");
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -1213,13 +1252,9 @@ int ses3d_hip_init(ses3d_hip_context *c) {
         }
         // sampled-step slots: adjoint_flag 1 stores, 2 reads the fwd step nt+1-it
         if (af == 1 || af == 2) {
-            TRY(build_slots(c, E, true));
-            std::vector<int> sl(nt + 1, -1);
-            for (int it = 1; it <= nt; ++it) {
-                if (af == 1) sl[it] = E.slot_h[it];
-                else sl[it] = E.slot_h[nt + 1 - it];
-            }
-            TRY(upload(c, E.slot_d, sl.data(), sizeof(int) * (nt + 1)));
+            TRY(build_slots(c, E, false));
+            E.hist = nullptr;
+            TRY(upload_slots(c, E, std::vector<int>(nt + 1, -1)));
             for (int it = 1; it <= nt; ++it) c->saving_vector[it] = E.slot_h[it] >= 0 ? it : 0;
         }
     }
@@ -1248,6 +1283,7 @@ int ses3d_hip_run(ses3d_hip_context *c, int32_t it_begin, int32_t n_steps) {
     if (it_begin < 1 || n_steps < 0 || it_begin + n_steps - 1 > c->cfg.nt)
         return fail(SES3D_HIP_EINVAL, "step range outside 1..nt");
     const int af = c->cfg.adjoint_flag;
+    if (af == 1 || af == 2) TRY(ensure_full_history(c));
     c->t_elem = c->t_node = c->t_total = 0;
     c->n_elem = 0;
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -1418,12 +1454,13 @@ int ses3d_hip_get_kernels(ses3d_hip_context *c, int32_t ev, double *g_rho, doubl
 int ses3d_hip_get_history(ses3d_hip_context *c, int32_t ev, int32_t step_, float *out9) {
     TRY(check_ev(c, ev));
     EventHost &E = c->evh[ev];
-    if (!E.hist || step_ < 1 || step_ > c->cfg.nt || E.slot_h.empty() || E.slot_h[step_] < 0)
+    if (!E.hist_own) return fail(SES3D_HIP_ESTATE, "no stored forward history (run an adjoint_flag=1 context first)");
+    if (step_ < 1 || step_ > c->cfg.nt || E.slot_h.empty() || E.slot_h[step_] < 0)
         return fail(SES3D_HIP_EINVAL, "step is not a stored forward step");
     const long long NP = c->NP;
     std::vector<float> tmp(NP);
     for (int k = 0; k < 9; ++k) {
-        TRY(download(c, tmp.data(), E.hist + ((long long)E.slot_h[step_] * 9 + k) * NP, sizeof(float) * NP));
+        TRY(download(c, tmp.data(), E.hist_own + ((long long)E.slot_h[step_] * 9 + k) * NP, sizeof(float) * NP));
         em_to_ref(c, tmp.data(), out9 + k * NP);
     }
     return 0;
@@ -1431,14 +1468,17 @@ int ses3d_hip_get_history(ses3d_hip_context *c, int32_t ev, int32_t step_, float
 
 int ses3d_hip_put_history(ses3d_hip_context *c, int32_t ev, int32_t step_, const float *in9) {
     TRY(check_ev(c, ev));
+    if (!c->initialised) return fail(SES3D_HIP_ESTATE, "init must precede put_history");
+    if (c->cfg.adjoint_flag != 2) return fail(SES3D_HIP_ESTATE, "put_history needs an adjoint_flag=2 context");
+    TRY(ensure_full_history(c));
     EventHost &E = c->evh[ev];
-    if (!E.hist || step_ < 1 || step_ > c->cfg.nt || E.slot_h.empty() || E.slot_h[step_] < 0)
-        return fail(SES3D_HIP_EINVAL, "step is not a sampled forward step (init the adjoint context first)");
+    if (step_ < 1 || step_ > c->cfg.nt || E.slot_h.empty() || E.slot_h[step_] < 0)
+        return fail(SES3D_HIP_EINVAL, "step is not a sampled forward step");
     const long long NP = c->NP;
     std::vector<float> tmp(NP);
     for (int k = 0; k < 9; ++k) {
         ref_to_em(c, in9 + k * NP, tmp.data());
-        TRY(upload(c, E.hist + ((long long)E.slot_h[step_] * 9 + k) * NP, tmp.data(), sizeof(float) * NP));
+        TRY(upload(c, E.hist_own + ((long long)E.slot_h[step_] * 9 + k) * NP, tmp.data(), sizeof(float) * NP));
     }
     return 0;
 }
@@ -1450,13 +1490,105 @@ int ses3d_hip_get_saving_vector(ses3d_hip_context *c, int32_t *sv) {
 }
 
 int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
-    (void)c; (void)k;
-    return fail(SES3D_HIP_ESTATE, "checkpointed forward not available in this build");
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (!c->initialised) return fail(SES3D_HIP_ESTATE, "init must precede forward_checkpointed");
+    if (c->cfg.adjoint_flag != 1) return fail(SES3D_HIP_ESTATE, "forward_checkpointed needs an adjoint_flag=1 context");
+    if (k < 1) return fail(SES3D_HIP_EINVAL, "checkpoint interval must be >= 1");
+    const int nt = c->cfg.nt;
+    const int nseg = (nt + k - 1) / k;
+    const size_t NP = (size_t)c->NP, NGB = (size_t)c->NGB;
+    const size_t nM = c->cfg.is_diss ? 18 * NP : 0;
+    const size_t ck = 3 * NGB + 18 * NP + nM;
+    // the forward pass keeps the displacement gradient (mode 1) but stores no sampled step
+    for (auto &E : c->evh) {
+        if ((int)E.ckpt.size() != nseg) {
+            E.ckpt.assign(nseg, nullptr);
+            for (int s = 0; s < nseg; ++s) TRY(dalloc(c, &E.ckpt[s], ck));
+        }
+        E.hist = nullptr;
+        TRY(upload_slots(c, E, std::vector<int>(nt + 1, -1)));
+    }
+    TRY(refresh_events(c));
+    c->ckpt_k = k;
+    for (int sgi = 0; sgi < nseg; ++sgi) {
+        for (auto &E : c->evh) {
+            float *dst = E.ckpt[sgi];
+            HIP_TRY(hipMemcpyAsync(dst, E.v, sizeof(float) * 3 * NGB, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(dst + 3 * NGB, E.d, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(dst + 3 * NGB + 9 * NP, E.s, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, c->stream));
+            if (nM) HIP_TRY(hipMemcpyAsync(dst + 3 * NGB + 18 * NP, E.M, sizeof(float) * nM, hipMemcpyDeviceToDevice, c->stream));
+        }
+        const int s0 = sgi * k + 1, s1 = std::min((sgi + 1) * k, nt);
+        for (int it = s0; it <= s1; ++it) TRY(step(c, it, 1, true));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->ckpt_ready = true;
+    return 0;
 }
 
 int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
-    (void)a; (void)f;
-    return fail(SES3D_HIP_ESTATE, "checkpointed adjoint not available in this build");
+    if (!a || !f) return fail(SES3D_HIP_EINVAL, "null context");
+    if (!a->initialised) return fail(SES3D_HIP_ESTATE, "init the adjoint context first");
+    if (a->cfg.adjoint_flag != 2) return fail(SES3D_HIP_ESTATE, "adjoint context needs adjoint_flag=2");
+    if (!f->ckpt_ready) return fail(SES3D_HIP_ESTATE, "run forward_checkpointed on the forward context first");
+    if (a->cfg.nt != f->cfg.nt || a->NE != f->NE || a->n_events != f->n_events || a->cfg.samp_ad != f->cfg.samp_ad)
+        return fail(SES3D_HIP_EINVAL, "forward and adjoint contexts differ in mesh, nt, samp_ad or event count");
+    const int nt = f->cfg.nt, k = f->ckpt_k;
+    const int nseg = (nt + k - 1) / k;
+    const size_t NP = (size_t)f->NP, NGB = (size_t)f->NGB;
+    const size_t nM = f->cfg.is_diss ? 18 * NP : 0;
+    // segment buffer: the sampled steps of the longest segment
+    int max_slots = 0;
+    {
+        const EventHost &E0 = f->evh[0];
+        for (int sgi = 0; sgi < nseg; ++sgi) {
+            int n = 0;
+            for (int it = sgi * k + 1; it <= std::min((sgi + 1) * k, nt); ++it) n += E0.slot_h[it] >= 0;
+            max_slots = std::max(max_slots, n);
+        }
+    }
+    for (auto &E : f->evh)
+        if (E.seg_slots < max_slots) {
+            TRY(dalloc(f, &E.seg, (size_t)max_slots * 9 * NP));
+            E.seg_slots = max_slots;
+        }
+    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+        const int s0 = sgi * k + 1, s1 = std::min((sgi + 1) * k, nt);
+        // restore the forward state at the start of the segment, replay it into the buffer
+        for (int ev = 0; ev < f->n_events; ++ev) {
+            EventHost &E = f->evh[ev];
+            const float *src = E.ckpt[sgi];
+            HIP_TRY(hipMemcpyAsync(E.v, src, sizeof(float) * 3 * NGB, hipMemcpyDeviceToDevice, f->stream));
+            HIP_TRY(hipMemcpyAsync(E.d, src + 3 * NGB, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, f->stream));
+            HIP_TRY(hipMemcpyAsync(E.s, src + 3 * NGB + 9 * NP, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, f->stream));
+            if (nM) HIP_TRY(hipMemcpyAsync(E.M, src + 3 * NGB + 18 * NP, sizeof(float) * nM, hipMemcpyDeviceToDevice, f->stream));
+            std::vector<int> tab(nt + 1, -1);
+            int n = 0;
+            for (int it = s0; it <= s1; ++it)
+                if (E.slot_h[it] >= 0) tab[it] = n++;
+            E.hist = E.seg;
+            TRY(upload_slots(f, E, tab));
+        }
+        TRY(refresh_events(f));
+        for (int it = s0; it <= s1; ++it) TRY(step(f, it, 1, false));
+        HIP_TRY(hipStreamSynchronize(f->stream));
+        // adjoint steps paired with this segment's forward steps
+        const int a0 = nt + 1 - s1, a1 = nt + 1 - s0;
+        for (int ev = 0; ev < a->n_events; ++ev) {
+            EventHost &A = a->evh[ev];
+            const EventHost &E = f->evh[ev];
+            std::vector<int> tab(nt + 1, -1);
+            for (int it = a0; it <= a1; ++it) tab[it] = E.slot_dev[nt + 1 - it];
+            A.hist = E.seg;
+            TRY(upload_slots(a, A, tab));
+        }
+        TRY(refresh_events(a));
+        for (int it = a0; it <= a1; ++it) TRY(step(a, it, 2, false));
+        HIP_TRY(hipStreamSynchronize(a->stream));
+    }
+    for (auto &A : a->evh) A.hist = A.hist_own;
+    TRY(refresh_events(a));
+    return 0;
 }
 
 int ses3d_hip_memory_bytes(ses3d_hip_context *c, int64_t *bytes) {

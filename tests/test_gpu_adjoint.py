@@ -1,0 +1,92 @@
+"""Adjoint path on the GPU: Frechet kernels (SOURCE/ses3d_grad.f90,
+ses3d_grad_Q.f90) from the stored-history path and from the HBM-checkpoint
+path.  The checkpoint path replays each forward segment from a restart state,
+which reproduces the stored forward fields bit for bit, so its kernels must
+equal the stored-history kernels exactly.  Against the float32 oracle the bar is
+relative L2 <= 1e-5 (float64 accumulation on the GPU vs float32 in the
+reference), and against the float64 oracle <= 1e-4."""
+import numpy as np
+import pytest
+
+from tests.cases import make_case, oracle_for
+
+pytestmark = pytest.mark.gpu
+KNAMES = ("grad_rho", "grad_cp", "grad_csh", "grad_csv", "grad_Q_mu")
+
+
+def _ctx(run, af, nt=None):
+    from ses3d_amd.solver import Solver
+    s = Solver.from_run(run, adjoint_flag=af)
+    s.set_model(run.model)
+    return s
+
+
+def _adjoint_sources(e, seis):
+    """L2-type adjoint sources at the located receivers, rows time-reversed as
+    TOOLS/adjoint_source.py:89 writes them."""
+    loc = e.rec32
+    adsrc = (seis.astype(np.float64) * 1e6).astype(np.float32)         # (3, nr, nt) forward time
+    rows = np.ascontiguousarray(adsrc[:, :, ::-1].transpose(1, 2, 0))  # (nr, nt, 3) reversed
+    return loc, rows
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("is_diss,k", [(1, 7), (0, 16)])
+def test_checkpointed_gradient(tmp_path, is_diss, k):
+    run = make_case(str(tmp_path), nel=(6, 6, 4), nt=60, is_diss=bool(is_diss), adjoint_flag=1, samp_ad=4,
+                    seed=13 + is_diss)
+    e = run.events[0]
+    nt = e.nt
+    print(f"case diss={is_diss} k={k}", flush=True)
+    # stored-history path
+    f = _ctx(run, 1)
+    f.load_event(0, e)
+    f.init()
+    f.run()
+    seis = f.seismograms()
+    loc, rows = _adjoint_sources(e, seis)
+    a = _ctx(run, 2)
+    a.set_adjoint_sources(0, loc, rows)
+    a.init()
+    for st in np.flatnonzero(f.saving_vector()) + 1:
+        a.put_history(int(st), f.history(int(st)))
+    a.run()
+    K1 = a.kernels()
+    # checkpoint path
+    f2 = _ctx(run, 1)
+    f2.load_event(0, e)
+    f2.init()
+    f2.forward_checkpointed(k)
+    np.testing.assert_array_equal(f2.seismograms(), seis)
+    a2 = _ctx(run, 2)
+    a2.set_adjoint_sources(0, loc, rows)
+    a2.init()
+    a2.adjoint_checkpointed(f2)
+    K2 = a2.kernels()
+    for g in K1:
+        np.testing.assert_array_equal(K2[g], K1[g], err_msg=g)
+    # oracle
+    of = oracle_for(run, e, adjoint_flag=1)
+    of.run(1, nt)
+    np.testing.assert_array_equal(of.get("seismograms").astype(np.float32), seis)
+    of64 = oracle_for(run, e, adjoint_flag=1, precision=64)
+    of64.run(1, nt)
+    for prec, ofw, tol in ((32, of, 1e-5), (64, of64, 1e-4)):
+        from oracle import pyoracle as PO
+        oa = PO.Oracle(nx=run.nx, ny=run.ny, nz=run.nz, xmin=run.xmin, xmax=run.xmax, ymin=run.ymin,
+                       ymax=run.ymax, zmin=run.zmin, zmax=run.zmax, zmax_global=run.zmax_global, nt=nt,
+                       dt=e.dt, is_diss=run.is_diss, adjoint_flag=2, samp_ad=run.samp_ad, pml=2,
+                       tau_p=run.tau_p, D_p=run.D_p, precision=prec)
+        oa.set_model(run.model)
+        oa.set_adjoint_sources(loc, rows)
+        oa.init()
+        oa.run(1, nt, fwd=ofw)
+        for g in K1:
+            ref = oa.get(g)
+            assert np.linalg.norm(ref) > 0, g
+            rel = _rel(K1[g], ref)
+            print(f"  {g} vs oracle{prec}: rel L2 {rel:.2e}", flush=True)
+            assert rel <= tol, (g, prec, rel)
