@@ -53,6 +53,22 @@ def algorithmic_bytes(nel, is_diss: bool) -> dict:
     return {"rho_g": rho_g, "step": step, "element": elem, "node": 4 * 4 * rho_g}
 
 
+def implemented_bytes(nel, pml: int, is_diss: bool, aniso: bool = False) -> float:
+    """Bytes per GLL point the implemented element kernel must move (forward
+    mode).  Elements outside every absorbing layer keep 6 split stresses and no
+    displacement gradient; elements in a layer keep 9 + 9.  Coefficients:
+    lambda, mu (+Q) (+A, B, C only for anisotropic models); the velocity is
+    gathered once per global node; the element force (3 floats) is written."""
+    nx, ny, nz = nel
+    n_p = nx * ny * nz * 125
+    rho_g = (4 * nx + 1) * (4 * ny + 1) * (4 * nz + 1) / n_p
+    inner = max(nx - 2 * pml, 0) * max(ny - 2 * pml, 0) * max(nz - pml, 0) / (nx * ny * nz)
+    mem = 36 if is_diss else 0
+    coef = 2 + (1 if is_diss else 0) + (3 if aniso else 0)
+    common = 4 * (mem + coef + 3 * rho_g + 3)
+    return inner * (common + 4 * 12) + (1 - inner) * (common + 4 * 36)
+
+
 def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
     """Time the reference binary on its largest one-rank mesh (its static
     arrays hold 23 x 28 x 8 elements, SOURCE/ses3d_modules.f90:13-15);
@@ -92,6 +108,82 @@ def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
                        f"({times[1]:.1f} s and {times[0]:.1f} s)")}
 
 
+def gradient_leg(k: int = 100, nt=None) -> dict:
+    """Seconds per event-gradient on the differentiation workload (48x48x24
+    anelastic, 100 receivers, dt=0.15): data from the 'true' model (1D + 2%
+    Gaussian blobs), gradient of the L2 waveform misfit at the 1D model by the
+    checkpointed forward (restart states every k steps in HBM) and the adjoint
+    with kernel accumulation.  The timed region is forward + adjoint."""
+    from dataclasses import replace
+    from ses3d_amd import configs, misfit
+    from ses3d_amd.solver import Solver
+    w = configs.WORKLOADS["gradient_48x48x24"]
+    nt = nt or w.nt
+    true = configs.build(w, seed=0, nt=nt)
+    cur = configs.build(replace(w, perturbation="none"), seed=0, nt=nt)
+    e = cur.events[0]
+    f0 = Solver.from_run(true, adjoint_flag=0, pml=w.pml, nt=nt)
+    f0.set_model(true.model)
+    f0.load_event(0, true.events[0])
+    f0.init()
+    f0.run()
+    data = f0.seismograms()
+    f0.close()
+    del true
+    f = Solver.from_run(cur, adjoint_flag=1, pml=w.pml, nt=nt)
+    f.set_model(cur.model)
+    f.load_event(0, e)
+    f.init()
+    a = Solver.from_run(cur, adjoint_flag=2, pml=w.pml, nt=nt)
+    a.set_model(cur.model)
+    t0 = time.perf_counter()
+    f.forward_checkpointed(k)
+    u = f.seismograms()
+    rows = misfit.l2_adjoint_rows(u, data, float(e.dt))
+    a.set_adjoint_sources(0, e.rec32[f.receiver_index(0)], rows)
+    a.init()
+    a.adjoint_checkpointed(f)
+    kern = a.kernels()
+    t1 = time.perf_counter()
+    ok = all(np.isfinite(v).all() and np.abs(v).max() > 0 for v in kern.values())
+    out = {"s_per_event_gradient": t1 - t0, "workload": f"gradient {w.name} anelastic nt={nt} dt={w.dt}",
+           "checkpoint_every": k, "receivers": int(f.nr(0)), "gll_points": w.n_points,
+           "misfit": misfit.l2_misfit(u, data, float(e.dt)), "kernels_finite_nonzero": bool(ok),
+           "device_GiB": (f.memory_bytes() + a.memory_bytes()) / 2 ** 30}
+    f.close()
+    a.close()
+    return out
+
+
+def batched_leg(n_events: int = 32, steps: int = 30) -> dict:
+    """Per-event forward throughput with n_events events advancing in one
+    launch per kernel per step on the 64x64x32 anelastic mesh, against one
+    event alone on the same mesh."""
+    from ses3d_amd import configs
+    from ses3d_amd.solver import Solver
+    w = configs.WORKLOADS["batched_64x64x32"]
+    nt = steps + 5
+    run = configs.build(w, seed=0, nt=nt, n_events=n_events)
+    res = {}
+    for n in (1, n_events):
+        s = Solver.from_run(run, adjoint_flag=0, pml=w.pml, nt=nt, n_events=n)
+        s.set_model(run.model)
+        for k in range(n):
+            s.load_event(k, run.events[k])
+        s.init()
+        s.run(1, 5)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.run(6, steps)
+        dt = time.perf_counter() - t0
+        res[n] = w.n_points * steps * n / dt
+        s.close()
+    return {"workload": f"forward {w.name} anelastic, {n_events} events (seeded sources 5-600 km deep, "
+                        f"10-400 receivers each)", "plain_updates_per_s": res[1],
+            "batched_updates_per_s": res[n_events], "batched_per_event_over_plain": res[n_events] / res[1],
+            "n_events": n_events, "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +191,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="anelastic_96x96x48")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gradient", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a device copy of this many MiB (for HBM counter calibration)")
     args = ap.parse_args()
@@ -168,11 +262,20 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_element", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "algorithmic_bytes_per_point": ab["element"], "points_per_launch": n_p,
-                     "mean_launch_ms": elem_ms},
+                     "mean_launch_ms": elem_ms,
+                     "impl_bytes_per_point": implemented_bytes(w.nel, w.pml, w.is_diss),
+                     "frac_impl": implemented_bytes(w.nel, w.pml, w.is_diss) * n_p / (elem_ms * 1e-3) / HBM_PEAK,
+                     "traffic_frac": (traffic / (elem_ms * 1e-3) / HBM_PEAK) if traffic else None},
         "step_roofline_frac": value * ab["step"] / HBM_PEAK,
         "kernel_ms_per_step": {"element": elem_ms, "node": tm["node_ms"] / max(tm["n_element"], 1)},
     }
     s.close()
+    if not args.no_gradient:
+        print("# gradient leg", file=sys.stderr, flush=True)
+        out["gradient"] = gradient_leg()
+    if not args.no_batched:
+        print("# batched leg", file=sys.stderr, flush=True)
+        out["batched"] = batched_leg()
     if not args.no_cpu_baseline:
         print("# timing the reference CPU solver", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline_reference(w.is_diss)

@@ -56,5 +56,32 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FC = os.environ.get("SES3D_FC", "/opt/rocm/lib/llvm/bin/amdflang")
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "ses3d_hip.exe")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the Fortran drop-in driver (iso_c_binding over libses3d_hip.so);
+    returns its path, or "" when no Fortran compiler is available."""
+    if not os.path.exists(FC):
+        return ""
+    objdir = os.path.join(HERE, "_build", "fortran")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in ("ses3d_hip_mod.f90", "ses3d_hip_main.f90"):
+        obj = os.path.join(objdir, src + ".o")
+        cmd = [FC, "-O2", "-J", objdir, "-c", os.path.join(FORTRAN, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(obj)
+    cmd = [FC, "-o", DRIVER + ".tmp", *objs, f"-L{HERE}", "-lses3d_hip", "-Wl,-rpath,$ORIGIN/.."]
+    _run(cmd)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

@@ -1,0 +1,36 @@
+"""L2 waveform misfit and its adjoint sources in SES3D's conventions.
+
+The reference's Python tools make adjoint sources from velocity seismograms
+(TOOLS/adjoint_source.py:323-394) and write them time-reversed
+(TOOLS/adjoint_source.py:84-91); the adjoint solver reads the rows in file
+order (SOURCE/ses3d_init.f90:486-490).  This module builds the same rows in
+memory for the plain waveform misfit
+
+    chi = 1/2 * sum_receivers sum_components sum_t (u(t) - d(t))^2 * dt
+
+of velocity seismograms u (the reference samples seismograms as velocity,
+SOURCE/ses3d_output.f90:485-487).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def l2_misfit(u: np.ndarray, d: np.ndarray, dt: float) -> float:
+    """chi for seismograms (3, nr, nt)."""
+    r = np.asarray(u, np.float64) - np.asarray(d, np.float64)
+    return 0.5 * float(np.sum(r * r)) * dt
+
+
+def l2_adjoint_rows(u: np.ndarray, d: np.ndarray, dt: float, scale: float = 1.0) -> np.ndarray:
+    """Adjoint-source rows (nr, nt, 3) float32, time-reversed as stored in
+    ADJOINT/<ev>/ad_src_<k>, for the residual u - d of velocity seismograms.
+
+    The residual is differentiated in time (-d/dt, last sample 0) because the
+    measurement is on velocity seismograms, the convention of
+    TOOLS/adjoint_source.py:383-387."""
+    r = (np.asarray(u, np.float64) - np.asarray(d, np.float64)) * scale   # (3, nr, nt)
+    a = np.zeros_like(r)
+    a[..., :-1] = -np.diff(r, axis=-1) / dt
+    rows = a[:, :, ::-1].transpose(1, 2, 0)
+    return np.ascontiguousarray(rows, np.float32)
