@@ -139,7 +139,8 @@ def gradient_leg(k: int = 100, nt=None) -> dict:
     t0 = time.perf_counter()
     f.forward_checkpointed(k)
     u = f.seismograms()
-    rows = misfit.l2_adjoint_rows(u, data, float(e.dt))
+    taper = misfit.cosine_taper(nt, float(e.dt), 40.0)
+    rows = misfit.l2_adjoint_rows(u, data, float(e.dt), taper=taper)
     a.set_adjoint_sources(0, e.rec32[f.receiver_index(0)], rows)
     a.init()
     a.adjoint_checkpointed(f)
@@ -148,7 +149,7 @@ def gradient_leg(k: int = 100, nt=None) -> dict:
     ok = all(np.isfinite(v).all() and np.abs(v).max() > 0 for v in kern.values())
     out = {"s_per_event_gradient": t1 - t0, "workload": f"gradient {w.name} anelastic nt={nt} dt={w.dt}",
            "checkpoint_every": k, "receivers": int(f.nr(0)), "gll_points": w.n_points,
-           "misfit": misfit.l2_misfit(u, data, float(e.dt)), "kernels_finite_nonzero": bool(ok),
+           "misfit": misfit.l2_misfit(u, data, float(e.dt), taper), "kernels_finite_nonzero": bool(ok),
            "device_GiB": (f.memory_bytes() + a.memory_bytes()) / 2 ** 30}
     f.close()
     a.close()

@@ -1,0 +1,198 @@
+"""Differentiable SES3D misfit: ``torch.autograd.Function`` over the GPU solver.
+
+forward:  model (rho, vp, vsh, vsv[, Q]) at the GLL points -> SES3D moduli
+          (mu = rho vsh^2, B = rho vsv^2 - mu, lambda = rho vp^2 - 2 mu, as
+          MODELS/SOURCE/add_perturbation.f90:512-514) -> checkpointed forward
+          solve of every event (restart states every ``ckpt`` steps in HBM) ->
+          velocity seismograms -> chi = sum_events 1/2 sum (u - d)^2 dt.
+backward: adjoint sources from the residuals (ses3d_amd.misfit, the
+          convention of TOOLS/adjoint_source.py), checkpointed adjoint solve with
+          kernel accumulation (SOURCE/ses3d_grad.f90, ses3d_grad_Q.f90), and the
+          map from SES3D kernels to the gradient of chi with respect to the
+          per-point model values:
+
+              dchi/dm_p = -K_p * w_l w_m w_n * Jac^2 * r_p^2 sin(theta_p)
+
+          (GLL quadrature weight of the point in spherical coordinates,
+          w_l w_m w_n Jac r^2 sin(theta), times the Jac the reference divides its
+          kernels by, SOURCE/ses3d_grad.f90:77-83; the sign follows from the
+          adjoint being run forward in reversed time).  K is K_rho, K_cp, K_csh,
+          K_csv; for Q the relative-perturbation kernel of ses3d_grad_Q.f90
+          (normalised with 2 mu / Q) is divided by Q.
+
+The kernels are continuous-adjoint, time-sampled quantities, so the gradient
+equals the derivative of the discrete misfit up to the time-discretisation
+error, O(omega dt); tests/test_gpu_autograd.py checks it against central finite
+differences and checks that the error falls when dt is halved.
+"""
+from __future__ import annotations
+
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import misfit as MF
+from . import models as M
+from .solver import Solver
+
+GLL_W = np.array([0.1000000000000, 0.5444444444444, 0.7111111111111, 0.5444444444444, 0.1000000000000])
+
+
+def quadrature_weights(run) -> np.ndarray:
+    """w_l w_m w_n Jac^2 r^2 sin(theta) per GLL point, reference layout, float64."""
+    box = run.box
+    x, y, z = M.gll_coords(box)
+    dx = (box.xmax - box.xmin) / (box.nx + 1)
+    dy = (box.ymax - box.ymin) / (box.ny + 1)
+    dz = (box.zmax - box.zmin) / (box.nz + 1)
+    jac = dx * dy * dz / 8.0
+    w3 = GLL_W[:, None, None] * GLL_W[None, :, None] * GLL_W[None, None, :]
+    X = x[:, None, None, :, None, None]
+    Z = z[None, None, :, None, None, :]
+    W = w3[None, None, None] * (jac * jac) * Z ** 2 * np.sin(X)
+    return np.ascontiguousarray(np.broadcast_to(W, (box.nx + 1, box.ny + 1, box.nz + 1, 5, 5, 5)))
+
+
+class Problem:
+    """Fixed geometry, sources, receivers and observed data for a set of events.
+
+    ``run``: a RunInput (geometry, relaxation, events); ``data``: one observed
+    seismogram array (3, nr, nt) per event, receivers in the order the solver
+    locates them.  Events are solved ``events_per_batch`` at a time in one
+    context (they advance in the same launches)."""
+
+    def __init__(self, run, data: Sequence[np.ndarray], pml: int = 2, ckpt: int = 100,
+                 events_per_batch: Optional[int] = None, q_model: Optional[np.ndarray] = None,
+                 taper_width: float = 0.0):
+        self.run = run
+        self.data = [np.asarray(d, np.float32) for d in data]
+        self.pml = pml
+        self.ckpt = ckpt
+        self.n_events = len(run.events)
+        self.events_per_batch = events_per_batch or self.n_events
+        self.W = quadrature_weights(run)
+        self.dt = float(run.events[0].dt)
+        self.is_diss = bool(run.is_diss)
+        self.q_model = q_model if q_model is not None else run.model.get("Q")
+        self.last_seismograms: List[np.ndarray] = []
+        nt = run.events[0].nt
+        self.taper = MF.cosine_taper(nt, self.dt, taper_width) if taper_width > 0 else None
+
+    def fields(self, rho, vp, vsh, vsv, Q=None) -> dict:
+        rho, vp, vsh, vsv = (np.asarray(a, np.float64) for a in (rho, vp, vsh, vsv))
+        mu = rho * vsh * vsh
+        out = {"rhoinv": 1.0 / rho, "mu": mu, "lambda": rho * vp * vp - 2.0 * mu,
+               "A": np.zeros_like(rho), "B": rho * vsv * vsv - mu, "C": np.zeros_like(rho)}
+        if self.is_diss:
+            out["Q"] = np.asarray(Q if Q is not None else self.q_model, np.float64)
+        return {k: np.asfortranarray(v.astype(np.float32)) for k, v in out.items()}
+
+    def _batches(self):
+        for b0 in range(0, self.n_events, self.events_per_batch):
+            yield list(range(b0, min(b0 + self.events_per_batch, self.n_events)))
+
+    def _solver(self, model, af, n):
+        s = Solver.from_run(self.run, adjoint_flag=af, pml=self.pml, n_events=n)
+        s.set_model(model)
+        return s
+
+    def misfit(self, model) -> float:
+        """Plain forward solve of every event -> chi (no state kept)."""
+        chi = 0.0
+        self.last_seismograms = []
+        for evs in self._batches():
+            s = self._solver(model, 0, len(evs))
+            for k, ev in enumerate(evs):
+                s.load_event(k, self.run.events[ev])
+            s.init()
+            s.run()
+            for k, ev in enumerate(evs):
+                u = s.seismograms(k)
+                self.last_seismograms.append(u)
+                chi += MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
+            s.close()
+        return chi
+
+    def misfit_and_gradient(self, model):
+        """chi and the SES3D kernels summed over events (float64 dict)."""
+        chi = 0.0
+        K = None
+        self.last_seismograms = []
+        for evs in self._batches():
+            f = self._solver(model, 1, len(evs))
+            for k, ev in enumerate(evs):
+                f.load_event(k, self.run.events[ev])
+            f.init()
+            f.forward_checkpointed(self.ckpt)
+            a = self._solver(model, 2, len(evs))
+            for k, ev in enumerate(evs):
+                u = f.seismograms(k)
+                self.last_seismograms.append(u)
+                chi += MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
+                rows = MF.l2_adjoint_rows(u, self.data[ev], self.dt, taper=self.taper)
+                a.set_adjoint_sources(k, self.run.events[ev].rec32[f.receiver_index(k)], rows)
+            a.init()
+            a.adjoint_checkpointed(f)
+            for k in range(len(evs)):
+                kk = a.kernels(k)
+                if K is None:
+                    K = {n: v.copy() for n, v in kk.items()}
+                else:
+                    for n in K:
+                        K[n] += kk[n]
+            f.close()
+            a.close()
+        return chi, K
+
+    def gradient_from_kernels(self, K: dict, model: dict) -> dict:
+        """dchi/d(rho, vp, vsh, vsv[, Q]) per GLL point from summed kernels."""
+        s = -self.W
+        g = {"rho": s * K["grad_rho"], "vp": s * K["grad_cp"], "vsh": s * K["grad_csh"], "vsv": s * K["grad_csv"]}
+        if self.is_diss and "grad_Q_mu" in K:
+            g["Q"] = s * K["grad_Q_mu"] / np.asarray(model["Q"], np.float64)
+        return g
+
+
+try:
+    import torch
+
+    class Ses3dMisfit(torch.autograd.Function):
+        """chi(rho, vp, vsh, vsv, Q) with the adjoint method as backward."""
+
+        @staticmethod
+        def forward(ctx, rho, vp, vsh, vsv, Q, problem: Problem):
+            arrs = [t.detach().to("cpu", torch.float64).numpy() for t in (rho, vp, vsh, vsv)]
+            q = Q.detach().to("cpu", torch.float64).numpy() if Q is not None else None
+            model = problem.fields(*arrs, Q=q)
+            if any(ctx.needs_input_grad[:5]):
+                # forward and adjoint per event batch: checkpoints of one batch
+                # at a time are held in HBM, whatever the number of events
+                chi, K = problem.misfit_and_gradient(model)
+                ctx.grads = problem.gradient_from_kernels(K, model)
+            else:
+                chi = problem.misfit(model)
+                ctx.grads = {}
+            ctx.meta = [(t.dtype, t.device) for t in (rho, vp, vsh, vsv)] + \
+                       [((Q.dtype, Q.device) if Q is not None else None)]
+            return torch.tensor(chi, dtype=torch.float64, device=rho.device)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            go = float(grad_out.detach().cpu())
+            out = []
+            for name, meta in zip(("rho", "vp", "vsh", "vsv", "Q"), ctx.meta):
+                if meta is None or name not in ctx.grads:
+                    out.append(None)
+                    continue
+                dtype, dev = meta
+                out.append(torch.from_numpy(go * ctx.grads[name]).to(device=dev, dtype=dtype))
+            return (*out, None)
+
+    def misfit(problem: Problem, rho, vp, vsh, vsv=None, Q=None):
+        """Differentiable misfit; vsv defaults to vsh (isotropic)."""
+        if vsv is None:
+            vsv = vsh
+        return Ses3dMisfit.apply(rho, vp, vsh, vsv, Q, problem)
+
+except ImportError:   # torch is optional for the numpy API above
+    pass

@@ -1,0 +1,89 @@
+"""The differentiable misfit (ses3d_amd.autograd) against central finite
+differences of the discrete misfit, along seeded smooth random directions
+(Gaussian random fields, 300 km correlation), h = 1e-3 relative.
+
+What the tolerances rest on:
+* the misfit is windowed by a cosine taper that vanishes at both ends of the
+  record, so the integration by parts behind the velocity adjoint source
+  (TOOLS/adjoint_source.py:383-387) has no boundary term;
+* the reference's kernels (SOURCE/ses3d_grad.f90:77-83) pair stored velocities
+  and strains a few time steps apart, an O(dt) error that is largest for the
+  density term: it must fall by about half when dt is halved;
+* in anelastic models the reference's kernel formulas are the elastic ones
+  (mu instead of the unrelaxed mu_tau = (1 + sum D_p / Q) mu), a bias of order
+  sum D_p / Q ~ 2% for the shear terms; its shear-Q kernel is research code
+  (ses3d_grad_Q.f90, off-diagonal terms not doubled), only its sign and order
+  of magnitude are checked.
+"""
+import numpy as np
+import pytest
+import torch
+from dataclasses import replace
+
+from ses3d_amd import configs, models as M
+
+pytestmark = pytest.mark.gpu
+LABELS = (("rho", (0,)), ("vp", (1,)), ("vsh", (2,)), ("vsv", (3,)), ("Q", (4,)), ("all", (0, 1, 2, 3)))
+
+
+def _fd(dt, is_diss, h=1e-3, t_end=80.0):
+    from ses3d_amd.autograd import Problem, misfit
+    from ses3d_amd.solver import Solver
+    nt = int(round(t_end / dt))
+    w = configs.Workload("fd", (10, 10, 5), is_diss, 2, dt, nt, 16, "blobs", period=25.0)
+    true = configs.build(w, seed=0)
+    cur = configs.build(replace(w, perturbation="none"), seed=0)
+    s = Solver.from_run(true, adjoint_flag=0, pml=2)
+    s.set_model(true.model)
+    s.load_event(0, true.events[0])
+    s.init()
+    s.run()
+    data = s.seismograms()
+    s.close()
+    prob = Problem(cur, [data], pml=2, ckpt=40, taper_width=20.0)
+    m = cur.model
+    rho0 = 1.0 / m["rhoinv"].astype(np.float64)
+    vs0 = np.sqrt(m["mu"] / rho0)
+    vp0 = np.sqrt((m["lambda"] + 2 * m["mu"]) / rho0)
+    q0 = m["Q"].astype(np.float64) if is_diss else None
+    base0 = [rho0, vp0, vs0, vs0.copy(), q0]
+    base = [torch.tensor(a, requires_grad=True) if a is not None else None for a in base0]
+    chi = misfit(prob, *base)
+    chi.backward()
+    grads = [b.grad.numpy() if b is not None else None for b in base]
+    dirs = [M.gaussian_random_field(cur.box, 70 + k, corr_len=300e3, rms=1.0) for k in range(5)]
+    out = {}
+    for label, use in LABELS:
+        if 4 in use and not is_diss:
+            continue
+        dm = [(dirs[k] * base0[k] if k in use else None) for k in range(5)]
+        pred = sum(float(np.sum(grads[k] * dm[k])) for k in use)
+        vals = []
+        for sgn in (1, -1):
+            t = [None if base0[k] is None else torch.tensor(base0[k] + (sgn * h * dm[k] if dm[k] is not None else 0))
+                 for k in range(5)]
+            vals.append(float(misfit(prob, *t)))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        out[label] = (fd, pred, abs(pred - fd) / abs(fd))
+        print(f"dt={dt} diss={is_diss} {label}: fd={fd:.6e} adjoint={pred:.6e} rel.err={out[label][2]:.2e}", flush=True)
+    return out
+
+
+def test_elastic_gradient_vs_finite_differences():
+    e1 = _fd(0.25, False)
+    e2 = _fd(0.125, False)
+    for p in ("vp", "vsh", "vsv"):
+        assert e1[p][2] <= 0.005, (p, e1[p])
+    assert e1["all"][2] <= 0.02, e1["all"]
+    assert e2["all"][2] <= e1["all"][2], (e1["all"], e2["all"])
+    assert e1["rho"][2] <= 0.3, e1["rho"]
+    assert e2["rho"][2] <= 0.6 * e1["rho"][2], (e1["rho"], e2["rho"])   # first order in dt
+
+
+def test_anelastic_gradient_vs_finite_differences():
+    e = _fd(0.25, True)
+    for p in ("vp", "vsv"):
+        assert e[p][2] <= 0.005, (p, e[p])
+    assert e["vsh"][2] <= 0.04, e["vsh"]
+    assert e["all"][2] <= 0.05, e["all"]
+    assert np.sign(e["Q"][0]) == np.sign(e["Q"][1]) and e["Q"][2] <= 0.5, e["Q"]
