@@ -168,6 +168,13 @@ int ses3d_hip_get_saving_vector(ses3d_hip_context *ctx, int32_t *sv);
 int ses3d_hip_forward_checkpointed(ses3d_hip_context *fwd, int32_t k_ckpt);
 int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *adj, ses3d_hip_context *fwd);
 
+/* Launch structure: the step number lives in HBM and each step is a fixed
+ * sequence of kernels, so `steps` consecutive steps are captured once into a
+ * hipGraph and replayed (default 16; 0 launches every kernel directly).  No
+ * counterpart in the reference (it has no launches); results are identical
+ * either way. */
+int ses3d_hip_set_graph_steps(ses3d_hip_context *ctx, int32_t steps);
+
 /* Device memory held by the context [bytes]. */
 int ses3d_hip_memory_bytes(ses3d_hip_context *ctx, int64_t *bytes);
 

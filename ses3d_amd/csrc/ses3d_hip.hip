@@ -138,9 +138,10 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // ---------------------------------------------------------------------------
 template <int DISS, int MODE, int ANISO>
 __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restrict__ mesh,
-                                                          const EventDev *__restrict__ evs, int it,
+                                                          const EventDev *__restrict__ evs, const int *itp,
                                                           int blocks_per_event) {
     const MeshDev &Ms = *mesh;
+    const int it = *itp;   // the step number lives in HBM so a captured step graph replays unchanged
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int ev = blk / blocks_per_event;
     const int eb = blk - ev * blocks_per_event;
@@ -481,7 +482,8 @@ struct ForceDev {
 
 __global__ void k_forces(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs,
                          const ForceGroup *__restrict__ groups, const ForceDev *__restrict__ forces,
-                         int nt, int it) {
+                         int nt, const int *itp) {
+    const int it = *itp;
     const ForceGroup G = groups[blockIdx.x];
     const int nd = threadIdx.x;
     if (nd >= 125) return;
@@ -507,8 +509,10 @@ __global__ void k_forces(const MeshDev *__restrict__ mesh, const EventDev *__res
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict__ mesh,
                                                        const EventDev *__restrict__ evs,
-                                                       int nodeblocks_per_event, int n_events) {
+                                                       int nodeblocks_per_event, int n_events, int *itp) {
     const MeshDev &Ms = *mesh;
+    // last kernel of a step: advance the step number (no kernel of this launch reads it)
+    if (blockIdx.x == 0 && threadIdx.x == 0) *itp = *itp + 1;
     // 32-bit index math: n_events * padded nodes stays below 2^31 (checked on the host)
     const int gt = xcd_remap(blockIdx.x, gridDim.x) * NODE_THREADS + threadIdx.x;
     const int nbt = gt >> 6;
@@ -597,6 +601,10 @@ __global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict
     v[2 * Ms.NGB + g] = tg * vz;
 }
 
+__global__ void k_set_step(int *itp, int it) {
+    if (threadIdx.x == 0) *itp = it;
+}
+
 // mass matrix (SOURCE/ses3d_init.f90:254-277): the same ordered gather as k_node
 __global__ __launch_bounds__(NODE_THREADS) void k_mass(MeshDev *mesh, float *MM, const float *w5) {
     const MeshDev &Ms = *mesh;
@@ -647,7 +655,8 @@ struct RecDev {
 };
 
 __global__ void k_receivers(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs,
-                            const RecDev *__restrict__ recs, int nrec, int nt, int it) {
+                            const RecDev *__restrict__ recs, int nrec, int nt, const int *itp) {
+    const int it = *itp;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = t / 3, comp = t - 3 * r;
     if (r >= nrec) return;
@@ -674,6 +683,13 @@ struct DevBuf {
 };
 
 }  // namespace
+
+struct GraphEntry {
+    int mode = 0;
+    bool record = false;
+    int steps = 0;
+    hipGraphExec_t exec = nullptr;
+};
 
 struct EventHost {
     int source_type = 0;
@@ -734,6 +750,11 @@ struct ses3d_hip_context {
     std::vector<int> saving_vector;   // 1..nt
     int ckpt_k = 0;
     bool ckpt_ready = false;
+    // step number in HBM, and captured step graphs
+    int *it_d = nullptr;
+    int it_host = -1;                 // value the device counter holds next (-1 unknown)
+    int graph_steps = 16;
+    std::vector<GraphEntry> graphs;
     std::vector<void *> allocs;
     int64_t bytes = 0;
     // timing
@@ -810,13 +831,13 @@ int zero(ses3d_hip_context *c, void *p, size_t bytes) {
 int elem_blocks(const ses3d_hip_context *c) { return (int)((c->NE + EPB - 1) / EPB); }
 long long node_blocks(const ses3d_hip_context *c) { return (long long)c->NBX * c->NBY * c->NBZ; }
 
-int launch_element(ses3d_hip_context *c, int it, int mode) {
+int launch_element(ses3d_hip_context *c, int mode) {
     const int bpe = elem_blocks(c);
     dim3 grid((unsigned)(bpe * c->n_events)), block(ELEM_THREADS);
     const int key = (c->cfg.is_diss ? 100 : 0) + mode * 10 + (c->aniso ? 1 : 0);
 #define LAUNCH_E(D, MO, AN)                                                                                   \
     case D * 100 + MO * 10 + AN:                                                                              \
-        hipLaunchKernelGGL((k_element<D, MO, AN>), grid, block, 0, c->stream, c->mesh_d, c->ev_d, it, bpe); \
+        hipLaunchKernelGGL((k_element<D, MO, AN>), grid, block, 0, c->stream, c->mesh_d, c->ev_d, c->it_d, bpe); \
         break;
     switch (key) {
         LAUNCH_E(0, 0, 0) LAUNCH_E(0, 0, 1) LAUNCH_E(0, 1, 0) LAUNCH_E(0, 1, 1) LAUNCH_E(0, 2, 0) LAUNCH_E(0, 2, 1)
@@ -832,7 +853,7 @@ int launch_node(ses3d_hip_context *c) {
     const long long nb = node_blocks(c);
     const long long threads = nb * 64 * c->n_events;
     dim3 grid((unsigned)((threads + NODE_THREADS - 1) / NODE_THREADS)), block(NODE_THREADS);
-    hipLaunchKernelGGL(k_node, grid, block, 0, c->stream, c->mesh_d, c->ev_d, (int)nb, c->n_events);
+    hipLaunchKernelGGL(k_node, grid, block, 0, c->stream, c->mesh_d, c->ev_d, (int)nb, c->n_events, c->it_d);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -840,16 +861,31 @@ int launch_node(ses3d_hip_context *c) {
 // One iteration of the reference time loop for all events.  With profiling on,
 // HIP events bracket the element and node kernels on the context's stream;
 // they are read back after the run, so the stream is never stalled.
+int launch_step(ses3d_hip_context *c, int mode, bool record, bool timed);
+
 int step(ses3d_hip_context *c, int it, int mode, bool record) {
+    const int nt = c->cfg.nt;
+    if (c->it_host != it) {
+        hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, c->stream, c->it_d, it);
+        HIP_TRY(hipGetLastError());
+    }
+    TRY(launch_step(c, mode, record, c->profiling));
+    c->it_host = it + 1;
+    (void)nt;
+    return 0;
+}
+
+// The kernels of one step, reading the step number from HBM (graph-capturable).
+int launch_step(ses3d_hip_context *c, int mode, bool record, bool timed) {
     const int nt = c->cfg.nt;
     if (record && c->nrec_total > 0) {
         const int threads = 3 * c->nrec_total;
         hipLaunchKernelGGL(k_receivers, dim3((threads + 127) / 128), dim3(128), 0, c->stream, c->mesh_d,
-                           c->ev_d, c->recs_d, c->nrec_total, nt, it);
+                           c->ev_d, c->recs_d, c->nrec_total, nt, c->it_d);
         HIP_TRY(hipGetLastError());
     }
     hipEvent_t *ev = nullptr;
-    if (c->profiling) {
+    if (timed) {
         const size_t need = 4 * (size_t)(c->n_elem + 1);
         while (c->tev.size() < need) {
             hipEvent_t e;
@@ -859,11 +895,11 @@ int step(ses3d_hip_context *c, int it, int mode, bool record) {
         ev = &c->tev[4 * (size_t)c->n_elem];
         HIP_TRY(hipEventRecord(ev[0], c->stream));
     }
-    TRY(launch_element(c, it, mode));
+    TRY(launch_element(c, mode));
     if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
     if (c->n_groups > 0) {
         hipLaunchKernelGGL(k_forces, dim3(c->n_groups), dim3(128), 0, c->stream, c->mesh_d, c->ev_d,
-                           c->groups_d, c->forces_d, nt, it);
+                           c->groups_d, c->forces_d, nt, c->it_d);
         HIP_TRY(hipGetLastError());
     }
     if (ev) HIP_TRY(hipEventRecord(ev[2], c->stream));
@@ -872,6 +908,44 @@ int step(ses3d_hip_context *c, int it, int mode, bool record) {
         HIP_TRY(hipEventRecord(ev[3], c->stream));
         c->n_elem += 1;
     }
+    return 0;
+}
+
+// Steps it_begin.. through graphs of graph_steps captured steps each (one graph
+// per (mode, record) pair, captured on first use), the tail eagerly.
+int run_steps(ses3d_hip_context *c, int it_begin, int n, int mode, bool record) {
+    int it = it_begin;
+    const int G = c->graph_steps;
+    if (G > 0 && !c->profiling && n >= G) {
+        GraphEntry *ge = nullptr;
+        for (auto &g : c->graphs)
+            if (g.mode == mode && g.record == record && g.steps == G) ge = &g;
+        if (!ge) {
+            GraphEntry g;
+            g.mode = mode; g.record = record; g.steps = G;
+            hipGraph_t graph;
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+            int rc = 0;
+            for (int k = 0; k < G && !rc; ++k) rc = launch_step(c, mode, record, false);
+            hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(SES3D_HIP_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+            HIP_TRY(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+            HIP_TRY(hipGraphDestroy(graph));
+            c->graphs.push_back(g);
+            ge = &c->graphs.back();
+        }
+        if (c->it_host != it) {
+            hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, c->stream, c->it_d, it);
+            HIP_TRY(hipGetLastError());
+        }
+        while (it + G <= it_begin + n) {
+            HIP_TRY(hipGraphLaunch(ge->exec, c->stream));
+            it += G;
+        }
+        c->it_host = it;
+    }
+    for (; it < it_begin + n; ++it) TRY(step(c, it, mode, record));
     return 0;
 }
 
@@ -1006,6 +1080,7 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
     if (cfg->is_diss) rc = rc ? rc : dalloc(c, &c->Q, NP);
     rc = rc ? rc : dalloc(c, &c->MM, NGB);
     rc = rc ? rc : dalloc(c, &c->mesh_d, 1);
+    rc = rc ? rc : dalloc(c, &c->it_d, 1);
     rc = rc ? rc : dalloc(c, &c->ev_d, n_events);
     for (int ev = 0; ev < n_events && !rc; ++ev) {
         EventHost &E = c->evh[ev];
@@ -1030,6 +1105,7 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
 void ses3d_hip_destroy(ses3d_hip_context *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     for (void *p : c->allocs) (void)hipFree(p);
     for (auto &e : c->tev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1272,6 +1348,9 @@ int ses3d_hip_init(ses3d_hip_context *c) {
         TRY(upload(c, c->forces_d, forces.data(), sizeof(ForceDev) * forces.size()));
     }
     TRY(refresh_events(c));
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+    c->it_host = -1;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->initialised = true;
     return 0;
@@ -1292,7 +1371,7 @@ int ses3d_hip_run(ses3d_hip_context *c, int32_t it_begin, int32_t n_steps) {
         HIP_TRY(hipEventCreate(&t1));
         HIP_TRY(hipEventRecord(t0, c->stream));
     }
-    for (int it = it_begin; it < it_begin + n_steps; ++it) TRY(step(c, it, af, af != 2));
+    TRY(run_steps(c, it_begin, n_steps, af, af != 2));
     if (c->profiling) {
         HIP_TRY(hipEventRecord(t1, c->stream));
         TRY(collect_timing(c));
@@ -1519,7 +1598,7 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
             if (nM) HIP_TRY(hipMemcpyAsync(dst + 3 * NGB + 18 * NP, E.M, sizeof(float) * nM, hipMemcpyDeviceToDevice, c->stream));
         }
         const int s0 = sgi * k + 1, s1 = std::min((sgi + 1) * k, nt);
-        for (int it = s0; it <= s1; ++it) TRY(step(c, it, 1, true));
+        TRY(run_steps(c, s0, s1 - s0 + 1, 1, true));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->ckpt_ready = true;
@@ -1570,7 +1649,7 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
             TRY(upload_slots(f, E, tab));
         }
         TRY(refresh_events(f));
-        for (int it = s0; it <= s1; ++it) TRY(step(f, it, 1, false));
+        TRY(run_steps(f, s0, s1 - s0 + 1, 1, false));
         HIP_TRY(hipStreamSynchronize(f->stream));
         // adjoint steps paired with this segment's forward steps
         const int a0 = nt + 1 - s1, a1 = nt + 1 - s0;
@@ -1583,7 +1662,7 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
             TRY(upload_slots(a, A, tab));
         }
         TRY(refresh_events(a));
-        for (int it = a0; it <= a1; ++it) TRY(step(a, it, 2, false));
+        TRY(run_steps(a, a0, a1 - a0 + 1, 2, false));
         HIP_TRY(hipStreamSynchronize(a->stream));
     }
     for (auto &A : a->evh) A.hist = A.hist_own;
@@ -1594,6 +1673,15 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
 int ses3d_hip_memory_bytes(ses3d_hip_context *c, int64_t *bytes) {
     if (!c || !bytes) return fail(SES3D_HIP_EINVAL, "null argument");
     *bytes = c->bytes;
+    return 0;
+}
+
+int ses3d_hip_set_graph_steps(ses3d_hip_context *c, int32_t steps) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (steps < 0) return fail(SES3D_HIP_EINVAL, "graph step count must be >= 0");
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+    c->graph_steps = steps;
     return 0;
 }
 

@@ -79,6 +79,7 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_adjoint_checkpointed": ([P, P], C.c_int),
         "ses3d_hip_memory_bytes": ([P, C.POINTER(C.c_int64)], C.c_int),
         "ses3d_hip_set_profiling": ([P, i32], C.c_int),
+        "ses3d_hip_set_graph_steps": ([P, i32], C.c_int),
         "ses3d_hip_get_timing": ([P, dp, dp, dp, ip], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -271,6 +272,9 @@ class Solver:
         b = C.c_int64()
         _check(self.lib.ses3d_hip_memory_bytes(self.h, C.byref(b)))
         return b.value
+
+    def set_graph_steps(self, steps: int):
+        _check(self.lib.ses3d_hip_set_graph_steps(self.h, int(steps)))
 
     def set_profiling(self, on: bool):
         _check(self.lib.ses3d_hip_set_profiling(self.h, int(bool(on))))
