@@ -175,6 +175,12 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *adj, ses3d_hip_context *fw
  * either way. */
 int ses3d_hip_set_graph_steps(ses3d_hip_context *ctx, int32_t steps);
 
+/* Implementation state after init (no counterpart in the reference):
+ * out[0] reciprocal quotients by tau_p in use, out[1] by 3 in use,
+ * out[2..5] float32 numerators failing the exhaustive check for tau_p[0..2]
+ * and 3, out[6] anisotropic terms in use, out[7] steps per captured graph. */
+int ses3d_hip_get_info(ses3d_hip_context *ctx, int64_t *out, int32_t n);
+
 /* Device memory held by the context [bytes]. */
 int ses3d_hip_memory_bytes(ses3d_hip_context *ctx, int64_t *bytes);
 

@@ -48,7 +48,13 @@ int fail(int code, const std::string &msg) {
             return fail(SES3D_HIP_EDEVICE, std::string(#expr ": ") + hipGetErrorString(_e));  \
     } while (0)
 
-constexpr int EPB = 2;      // elements per block, element kernel
+#ifndef SES3D_EPB
+#define SES3D_EPB 2
+#endif
+#ifndef SES3D_WAVES_PER_EU
+#define SES3D_WAVES_PER_EU 0
+#endif
+constexpr int EPB = SES3D_EPB;   // elements per block, element kernel
 constexpr int TPE = 128;    // threads per element (125 active)
 constexpr int ELEM_THREADS = EPB * TPE;
 constexpr int NODE_THREADS = 256;   // 4 node blocks of 64 per workgroup
@@ -70,6 +76,11 @@ struct MeshDev {
     const unsigned short *IX, *IY, *IZ;
     const float *LUT;
     const unsigned char *PMLX, *PMLY, *PMLZ;   // per element index: inside an absorbing layer
+    // quotients by the relaxation times and by 3 through a reciprocal and two
+    // FMAs, enabled only for divisors whose every float32 numerator was
+    // checked at init to give the IEEE quotient bit for bit
+    float rtau_p[3], r3;
+    int fast_tau, fast_3;
 };
 
 struct EventDev {
@@ -89,25 +100,49 @@ struct EventDev {
 
 // Loads and stores of the streamed per-point state (touched once per step).
 // SES3D_NT_LOAD / SES3D_NT_STORE select non-temporal cache hints for them.
+// Pointers to HBM are cast to the global address space so loads and stores
+// compile to global_* instructions (counted by vmcnt only): LDS accesses then
+// need not wait for outstanding HBM loads.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const float cgfloat;
+#define GLOB(p) ((gfloat *)(p))
+#define CGLOB(p) ((cgfloat *)(p))
+
 #ifndef SES3D_NT_LOAD
 #define SES3D_NT_LOAD 0
 #endif
 #ifndef SES3D_NT_STORE
 #define SES3D_NT_STORE 0
 #endif
+template <typename T>
+__device__ __forceinline__ T ldg(const T *p) {
+    return *((__attribute__((address_space(1))) const T *)p);
+}
+
 __device__ __forceinline__ float ldnt(const float *p) {
 #if SES3D_NT_LOAD
-    return __builtin_nontemporal_load(p);
+    return __builtin_nontemporal_load(CGLOB(p));
 #else
-    return *p;
+    return *CGLOB(p);
 #endif
 }
 __device__ __forceinline__ void stnt(float *p, float v) {
 #if SES3D_NT_STORE
-    __builtin_nontemporal_store(v, p);
+    __builtin_nontemporal_store(v, GLOB(p));
 #else
-    *p = v;
+    *GLOB(p) = v;
 #endif
+}
+
+// a/b from y = RN(1/b): q = RN(a y), exact remainder, one correction step
+// (Markstein).  Below |a| = 2^-96 the remainder can underflow and the IEEE
+// division is used.  Enabled only for divisors for which this function was
+// checked against a / b for every float32 numerator.
+__device__ __forceinline__ float div_recip(float a, float b, float y) {
+    if (!(__builtin_fabsf(a) >= 0x1p-96f)) return a / b;
+    const float q = a * y;
+    const float r = __builtin_fmaf(-q, b, a);
+    return __builtin_fmaf(r, y, q);
 }
 
 __device__ __forceinline__ long long gidx(int gx, int gy, int gz, int NBX, int NBY) {
@@ -137,7 +172,11 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // absorbing terms, so it is integrated only inside the layers.
 // ---------------------------------------------------------------------------
 template <int DISS, int MODE, int ANISO>
-__global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restrict__ mesh,
+__global__ __launch_bounds__(ELEM_THREADS)
+#if SES3D_WAVES_PER_EU
+__attribute__((amdgpu_waves_per_eu(SES3D_WAVES_PER_EU, 8)))
+#endif
+void k_element(const MeshDev *__restrict__ mesh,
                                                           const EventDev *__restrict__ evs, const int *itp,
                                                           int blocks_per_event) {
     const MeshDev &Ms = *mesh;
@@ -156,7 +195,7 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
     __shared__ float tabs[7 * 25];
     __shared__ float sv[EPB][3][TPE];
     __shared__ float ss[EPB][9][TPE];
-    if (t < 7 * 25) tabs[t] = Ms.tab[t];
+    for (int k = t; k < 7 * 25; k += ELEM_THREADS) tabs[k] = ldg(Ms.tab + k);
     const float *D1 = tabs, *D2 = tabs + 25, *D3 = tabs + 50, *WDL = tabs + 75;
     const float *DUM1 = tabs + 100, *DUM2 = tabs + 125, *DUM3 = tabs + 150;
 
@@ -165,7 +204,7 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
     const int ei = e % NX, ej = (e / NX) % NY, ek = e / (NX * NY);
     const int p = e * 125 + nd;
     // element in an absorbing layer? (uniform over the element's wavefronts)
-    const bool pml = act && (Ms.PMLX[ei] | Ms.PMLY[ej] | Ms.PMLZ[ek]);
+    const bool pml = act && (ldg(Ms.PMLX + ei) | ldg(Ms.PMLY + ej) | ldg(Ms.PMLZ + ek));
     const bool keep_d = (MODE != 0) || pml;
 
     // All loads of the step are issued before the first barrier.
@@ -175,9 +214,9 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
     if (act) {
         // gather v (SOURCE/ses3d_evolution.f90:27-51)
         const long long g = gidx(4 * ei + l, 4 * ej + m, 4 * ek + n, Ms.NBX, Ms.NBY);
-        vx = E.v[g];
-        vy = E.v[Ms.NGB + g];
-        vz = E.v[2 * Ms.NGB + g];
+        vx = ldnt(E.v + g);
+        vy = ldnt(E.v + Ms.NGB + g);
+        vz = ldnt(E.v + 2 * Ms.NGB + g);
         if (keep_d) {
 #pragma unroll
             for (int c = 0; c < 9; ++c) d9[c] = ldnt(E.d + c * NP + p);
@@ -230,7 +269,7 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
             exz = exz + sv[sub][2][qx] * c1;
         }
         // spherical completion (:93-103)
-        const float r = Ms.Z[ek * 5 + n], st = Ms.S[ei * 5 + l], ct = Ms.COT[ei * 5 + l];
+        const float r = ldg(Ms.Z + ek * 5 + n), st = ldg(Ms.S + ei * 5 + l), ct = ldg(Ms.COT + ei * 5 + l);
         const float rst = r * st;
         ezz = -ezz;
         eyy = eyy / rst + (vz + vx * ct) / r;
@@ -261,9 +300,9 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
         // absorbing-layer terms (:123-134); exact no-ops outside the layers
         float px = 0.f, py = 0.f, pz = 0.f, taper = 1.f;
         if (pml) {
-            const unsigned ix = Ms.IX[ei * 5 + l], iy = Ms.IY[ej * 5 + m], iz = Ms.IZ[ek * 5 + n];
+            const unsigned ix = ldg(Ms.IX + ei * 5 + l), iy = ldg(Ms.IY + ej * 5 + m), iz = ldg(Ms.IZ + ek * 5 + n);
             const float *lut = Ms.LUT + (((size_t)iz * Ms.KY + iy) * Ms.KX + ix) * 5;
-            px = lut[0]; py = lut[1]; pz = lut[2]; taper = lut[4];
+            px = ldg(lut); py = ldg(lut + 1); pz = ldg(lut + 2); taper = ldg(lut + 4);
             exx = exx + (pz + py) * dxux;
             eyy = eyy + (px + pz) * dyuy;
             ezz = ezz + (px + py) * dzuz;
@@ -288,7 +327,10 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
             float Mdz = ((0.f + Mv[6]) + Mv[7]) + Mv[8];
             const float Md = (Mdx + Mdy) + Mdz;
             const float mt2 = (2.0f * mu) * tau;
-            const float L = (kappa - (2.0f * mut) / 3.0f) * ((exx + eyy) + ezz) - (mt2 * Md) / 3.0f;
+            const float t1 = 2.0f * mut, t2 = mt2 * Md;
+            const float d1 = Ms.fast_3 ? div_recip(t1, 3.0f, Ms.r3) : t1 / 3.0f;
+            const float d2 = Ms.fast_3 ? div_recip(t2, 3.0f, Ms.r3) : t2 / 3.0f;
+            const float L = (kappa - d1) * ((exx + eyy) + ezz) - d2;
             sxx = (L + (2.0f * mut) * exx) + mt2 * Mdx;
             syy = (L + (2.0f * mut) * eyy) + mt2 * Mdy;
             szz = (L + (2.0f * mut) * ezz) + mt2 * Mdz;
@@ -314,7 +356,9 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const float Mo = Mv[c * 3 + k];
-                    const float Mn = (Mo - (dt * Mo) / Ms.tau_p[k]) - Ms.Lk[k] * ev6[c];
+                    const float dM = dt * Mo;
+                    const float q = Ms.fast_tau ? div_recip(dM, Ms.tau_p[k], Ms.rtau_p[k]) : dM / Ms.tau_p[k];
+                    const float Mn = (Mo - q) - Ms.Lk[k] * ev6[c];
                     Mv[c * 3 + k] = Mn;
                     stnt(E.M + (c * 3 + k) * NP + p, Mn);
                 }
@@ -361,12 +405,12 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
         const int ncomp = pml ? 9 : 6;
         const int slot6[9] = {EXX, EYY, EZZ, EXY, EXZ, EYZ, EYX, EZX, EZY};
         if (e == E.src_elem) {
-            const float so = E.so[it - 1];
+            const float so = ldg(E.so + it - 1);
 #pragma unroll
             for (int k = 0; k < 9; ++k)
                 if (k < ncomp) {
                     const int c = slot6[k];
-                    ss[sub][c][nd] = s9[c] + (0.0f - E.src_md[c * 125 + nd] * so);
+                    ss[sub][c][nd] = s9[c] + (0.0f - ldg(E.src_md + c * 125 + nd) * so);
                 }
         } else {
 #pragma unroll
@@ -385,7 +429,7 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
 
         if (MODE == 1) {
             // forward-field store (ses3d_main.f90:237-247)
-            const int sl = E.slot[it];
+            const int sl = ldg(E.slot + it);
             if (sl >= 0) {
                 float *h = E.hist + (size_t)sl * 9 * NP;
                 h[0 * NP + p] = vx; h[1 * NP + p] = vy; h[2 * NP + p] = vz;
@@ -398,7 +442,7 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
         if (MODE == 2) {
             // kernels (SOURCE/ses3d_grad.f90:53-83, ses3d_grad_Q.f90:115-126); float32
             // increments in the reference's order, accumulated in float64
-            const int sl = E.slot[it];
+            const int sl = ldg(E.slot + it);
             if (sl >= 0) {
                 const float *h = E.hist + (size_t)sl * 9 * NP;
                 const float vxf = h[p], vyf = h[NP + p], vzf = h[2 * NP + p];
@@ -408,7 +452,7 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
                 const float tra = (dxux + dyuy) + dzuz, trf = (exxf + eyyf) + ezzf;
                 const float ede = ((((dxux * exxf + dyuy * eyyf) + dzuz * ezzf) + (2.0f * exya) * exyf) +
                                    (2.0f * exza) * exzf) + (2.0f * eyza) * eyzf;
-                const float rhoinv = Ms.rhoinv[p];
+                const float rhoinv = ldg(Ms.rhoinv + p);
                 const float rho = rhoinv > 0.f ? 1.0f / rhoinv : 0.f;
                 const float cs = sqrtf(mu * rhoinv);
                 const float cp = sqrtf((lam + 2.0f * mu) * rhoinv);
@@ -445,13 +489,13 @@ __global__ __launch_bounds__(ELEM_THREADS) void k_element(const MeshDev *__restr
         // weak-form stress divergence (:237-270); outside the layers the second
         // member of each stress pair is read from the first
         const int cyx = pml ? EYX : EXY, czx = pml ? EZX : EXZ, czy = pml ? EZY : EYZ;
-        const float zc = Ms.Z[ek * 5 + n], sa = Ms.S[ei * 5 + l];
+        const float zc = ldg(Ms.Z + ek * 5 + n), sa = ldg(Ms.S + ei * 5 + l);
         float ax = 0.f, ay = 0.f, az = 0.f;
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            const float wx = ((WDL[l * 5 + q] * zc) * Ms.S[ei * 5 + q]) * DUM1[m * 5 + n];
+            const float wx = ((WDL[l * 5 + q] * zc) * ldg(Ms.S + ei * 5 + q)) * DUM1[m * 5 + n];
             const float wy = (WDL[m * 5 + q] * zc) * DUM2[l * 5 + n];
-            const float zq = Ms.Z[ek * 5 + q];
+            const float zq = ldg(Ms.Z + ek * 5 + q);
             const float wz = (((WDL[n * 5 + q] * zq) * zq) * sa) * DUM3[l * 5 + m];
             const int pz_ = l + 5 * (m + 5 * q), py_ = l + 5 * (q + 5 * n), px_ = q + 5 * (m + 5 * n);
             ax = ((ax + ss[sub][czx][pz_] * wz) + ss[sub][cyx][py_] * wy) + ss[sub][EXX][px_] * wx;
@@ -581,24 +625,42 @@ __global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict
     {
         const int exl = bx < NX ? bx : NX - 1, eyl = by < NY ? by : NY - 1, ezl = bz < NZ ? bz : NZ - 1;
         const int lxl = 4 * bx + a - 4 * exl, lyl = 4 * by + b - 4 * eyl, lzl = 4 * bz + c - 4 * ezl;
-        const unsigned ix = Ms.IX[exl * 5 + lxl], iy = Ms.IY[eyl * 5 + lyl], iz = Ms.IZ[ezl * 5 + lzl];
+        const unsigned ix = ldg(Ms.IX + exl * 5 + lxl), iy = ldg(Ms.IY + eyl * 5 + lyl), iz = ldg(Ms.IZ + ezl * 5 + lzl);
         if (ix | iy | iz) {
             const float *lut = Ms.LUT + (((size_t)iz * Ms.KY + iy) * Ms.KX + ix) * 5;
-            pg = lut[3]; tg = lut[4];
+            pg = ldg(lut + 3); tg = ldg(lut + 4);
         }
     }
     const int g = (nb << 6) + loc;
-    const float mm = Ms.MM[g];
+    const float mm = ldnt(Ms.MM + g);
     const float dt = Ms.dt;
     float *v = E.v;
     const float ax = Sx / mm, ay = Sy / mm, az = Sz / mm;
-    float vx = v[g], vy = v[Ms.NGB + g], vz = v[2 * Ms.NGB + g];
+    float vx = ldnt(v + g), vy = ldnt(v + Ms.NGB + g), vz = ldnt(v + 2 * Ms.NGB + g);
     vx = vx + dt * (ax - pg * vx);
     vy = vy + dt * (ay - pg * vy);
     vz = vz + dt * (az - pg * vz);
-    v[g] = tg * vx;
-    v[Ms.NGB + g] = tg * vy;
-    v[2 * Ms.NGB + g] = tg * vz;
+    stnt(v + g, tg * vx);
+    stnt(v + Ms.NGB + g, tg * vy);
+    stnt(v + 2 * Ms.NGB + g, tg * vz);
+}
+
+// Count, for each divisor b[d], the float32 numerators a (all 2^32 bit
+// patterns except NaN/Inf) for which div_recip(a, b, y) differs from a / b.
+__global__ void k_verify_div(const float *b, const float *y, int nd, unsigned long long *bad) {
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (int d = 0; d < nd; ++d) {
+        const float bd = b[d], yd = y[d];
+        unsigned long long nbad = 0;
+        for (unsigned long long u = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; u < (1ULL << 32); u += stride) {
+            const float a = __uint_as_float((unsigned)u);
+            if (!__builtin_isfinite(a)) continue;
+            const float ref = a / bd;
+            const float fast = div_recip(a, bd, yd);
+            nbad += (__float_as_uint(ref) != __float_as_uint(fast)) && !(ref == 0.0f && fast == 0.0f);
+        }
+        if (nbad) atomicAdd(&bad[d], nbad);
+    }
 }
 
 __global__ void k_set_step(int *itp, int it) {
@@ -754,6 +816,7 @@ struct ses3d_hip_context {
     int *it_d = nullptr;
     int it_host = -1;                 // value the device counter holds next (-1 unknown)
     int graph_steps = 16;
+    unsigned long long div_bad[4] = {0, 0, 0, 0};   // numerators failing the reciprocal check
     std::vector<GraphEntry> graphs;
     std::vector<void *> allocs;
     int64_t bytes = 0;
@@ -1226,6 +1289,32 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     M.MM = c->MM; M.Z = c->Z; M.S = c->S; M.COT = c->COT; M.IX = c->IX; M.IY = c->IY; M.IZ = c->IZ;
     M.LUT = c->LUT;
     M.PMLX = c->PMLX; M.PMLY = c->PMLY; M.PMLZ = c->PMLZ;
+    {
+        // reciprocal-based quotients: verified exhaustively on this device
+        float bh[4] = {c->cfg.tau_p[0], c->cfg.tau_p[1], c->cfg.tau_p[2], 3.0f};
+        float yh[4];
+        for (int k = 0; k < 4; ++k) yh[k] = 1.0f / bh[k];
+        float *bd = nullptr, *yd = nullptr;
+        unsigned long long *badd = nullptr;
+        TRY(dalloc(c, &bd, 4));
+        TRY(dalloc(c, &yd, 4));
+        TRY(dalloc(c, &badd, 4));
+        TRY(upload(c, bd, bh, sizeof(bh)));
+        TRY(upload(c, yd, yh, sizeof(yh)));
+        HIP_TRY(hipMemsetAsync(badd, 0, sizeof(unsigned long long) * 4, c->stream));
+        hipLaunchKernelGGL(k_verify_div, dim3(8192), dim3(256), 0, c->stream, bd, yd, 4, badd);
+        HIP_TRY(hipGetLastError());
+        unsigned long long bad[4];
+        TRY(download(c, bad, badd, sizeof(bad)));
+        for (int k = 0; k < 3; ++k) M.rtau_p[k] = yh[k];
+        M.r3 = yh[3];
+        M.fast_tau = c->cfg.is_diss && bad[0] == 0 && bad[1] == 0 && bad[2] == 0;
+        M.fast_3 = bad[3] == 0;
+#ifdef SES3D_NO_RECIP_DIV
+        M.fast_tau = M.fast_3 = 0;
+#endif
+        c->div_bad[0] = bad[0]; c->div_bad[1] = bad[1]; c->div_bad[2] = bad[2]; c->div_bad[3] = bad[3];
+    }
     TRY(upload(c, c->mesh_d, &M, sizeof(MeshDev)));
     // mass matrix
     {
@@ -1673,6 +1762,14 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
 int ses3d_hip_memory_bytes(ses3d_hip_context *c, int64_t *bytes) {
     if (!c || !bytes) return fail(SES3D_HIP_EINVAL, "null argument");
     *bytes = c->bytes;
+    return 0;
+}
+
+int ses3d_hip_get_info(ses3d_hip_context *c, int64_t *out, int32_t n) {
+    if (!c || !out) return fail(SES3D_HIP_EINVAL, "null argument");
+    const int64_t v[8] = {c->mesh_h.fast_tau, c->mesh_h.fast_3, (int64_t)c->div_bad[0], (int64_t)c->div_bad[1],
+                          (int64_t)c->div_bad[2], (int64_t)c->div_bad[3], c->aniso ? 1 : 0, c->graph_steps};
+    for (int k = 0; k < n && k < 8; ++k) out[k] = v[k];
     return 0;
 }
 

@@ -80,6 +80,7 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_memory_bytes": ([P, C.POINTER(C.c_int64)], C.c_int),
         "ses3d_hip_set_profiling": ([P, i32], C.c_int),
         "ses3d_hip_set_graph_steps": ([P, i32], C.c_int),
+        "ses3d_hip_get_info": ([P, C.POINTER(C.c_int64), i32], C.c_int),
         "ses3d_hip_get_timing": ([P, dp, dp, dp, ip], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -272,6 +273,12 @@ class Solver:
         b = C.c_int64()
         _check(self.lib.ses3d_hip_memory_bytes(self.h, C.byref(b)))
         return b.value
+
+    def info(self) -> dict:
+        out = (C.c_int64 * 8)()
+        _check(self.lib.ses3d_hip_get_info(self.h, out, 8))
+        return {"fast_div_tau": bool(out[0]), "fast_div_3": bool(out[1]), "div_check_failures": list(out[2:6]),
+                "anisotropic": bool(out[6]), "graph_steps": int(out[7])}
 
     def set_graph_steps(self, steps: int):
         _check(self.lib.ses3d_hip_set_graph_steps(self.h, int(steps)))

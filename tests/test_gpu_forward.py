@@ -121,3 +121,15 @@ def test_graph_replays_match_eager(tmp_path):
     _assert_same(g.seismograms(), d.seismograms(), "seismograms")
     for st in (1, 35, 70):
         _assert_same(g.history(st), d.history(st), f"history {st}")
+
+
+def test_reciprocal_division_check(tmp_path):
+    """The reciprocal quotients are enabled only when the exhaustive check over
+    all float32 numerators found no difference from IEEE division; the
+    anelastic bit-exact tests above run with whatever the check decided."""
+    run = make_case(str(tmp_path), nel=(4, 4, 3), nt=5, is_diss=True)
+    s = _gpu(run)
+    info = s.info()
+    print("division check:", info, flush=True)
+    assert info["fast_div_tau"] == (sum(info["div_check_failures"][:3]) == 0)
+    assert info["fast_div_3"] == (info["div_check_failures"][3] == 0)
