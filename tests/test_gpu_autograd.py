@@ -87,3 +87,43 @@ def test_anelastic_gradient_vs_finite_differences():
     assert e["vsh"][2] <= 0.04, e["vsh"]
     assert e["all"][2] <= 0.05, e["all"]
     assert np.sign(e["Q"][0]) == np.sign(e["Q"][1]) and e["Q"][2] <= 0.5, e["Q"]
+
+
+def test_steepest_descent_update_lowers_misfit():
+    """One steepest-descent model update from the gradient of two events solved
+    together in one context lowers the misfit, judged by plain forward solves of
+    one event at a time (the acceptance check of an inversion iteration, at a
+    small size)."""
+    from ses3d_amd.autograd import Problem, misfit
+    from ses3d_amd.solver import Solver
+    w = configs.Workload("sd", (10, 10, 5), True, 2, 0.25, 320, 16, "blobs", period=25.0, n_events=2)
+    true = configs.build(w, seed=1)
+    cur = configs.build(replace(w, perturbation="none"), seed=1)
+    data = []
+    for k in range(2):
+        s = Solver.from_run(true, adjoint_flag=0, pml=2, n_events=1)
+        s.set_model(true.model)
+        s.load_event(0, true.events[k])
+        s.init()
+        s.run()
+        data.append(s.seismograms())
+        s.close()
+    prob = Problem(cur, data, pml=2, ckpt=40, taper_width=20.0, events_per_batch=2)
+    m = cur.model
+    rho0 = 1.0 / m["rhoinv"].astype(np.float64)
+    vs0 = np.sqrt(m["mu"] / rho0)
+    vp0 = np.sqrt((m["lambda"] + 2 * m["mu"]) / rho0)
+    x = [torch.tensor(a, requires_grad=True) for a in (rho0, vp0, vs0, vs0.copy())]
+    chi0 = misfit(prob, *x)
+    chi0.backward()
+    g = [t.grad.numpy() for t in x]
+    # step: 0.5% of the velocities along the steepest descent of vp, vsh, vsv
+    scale = [0.0, 0.005 * vp0.mean(), 0.005 * vs0.mean(), 0.005 * vs0.mean()]
+    new = [x[k].detach().numpy() - scale[k] * g[k] / max(np.abs(g[k]).max(), 1e-300) for k in range(4)]
+    # judge with plain single-event solves
+    judge = Problem(cur, data, pml=2, ckpt=40, taper_width=20.0, events_per_batch=1)
+    chi_before = judge.misfit(judge.fields(rho0, vp0, vs0, vs0))
+    chi_after = judge.misfit(judge.fields(*new))
+    print(f"misfit before {chi_before:.6e} after {chi_after:.6e}", flush=True)
+    assert abs(chi_before - float(chi0.detach())) <= 1e-6 * chi_before
+    assert chi_after < chi_before
