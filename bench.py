@@ -156,33 +156,71 @@ def gradient_leg(k: int = 100, nt=None) -> dict:
     return out
 
 
+def _timed_run(x, it0: int, steps: int) -> float:
+    x.synchronize()
+    t0 = time.perf_counter()
+    x.run(it0, steps, sync=False)
+    x.synchronize()
+    return time.perf_counter() - t0
+
+
 def batched_leg(n_events: int = 32, steps: int = 30) -> dict:
-    """Per-event forward throughput with n_events events advancing in one
-    launch per kernel per step on the 64x64x32 anelastic mesh, against one
-    event alone on the same mesh."""
+    """Uneven batch: n_events events on the 64x64x32 anelastic mesh plus six
+    extra anelastic sections (16^3 ... 56x56x28, one event each) advanced as one
+    group, one element-kernel and one node-kernel launch per step (autotuned
+    schedule, captured step graphs).  Compared with one event alone on the
+    64x64x32 mesh (per-event throughput) and with the same sections stepped one
+    context after another."""
     from ses3d_amd import configs
-    from ses3d_amd.solver import Solver
+    from ses3d_amd.solver import Group, Solver
     w = configs.WORKLOADS["batched_64x64x32"]
-    nt = steps + 5
-    run = configs.build(w, seed=0, nt=nt, n_events=n_events)
-    res = {}
-    for n in (1, n_events):
-        s = Solver.from_run(run, adjoint_flag=0, pml=w.pml, nt=nt, n_events=n)
+    tune, warm = 12, 16                    # autotune steps, then steps that capture the graph
+    nt = tune + warm + steps
+    runs = [(configs.build(w, seed=0, nt=nt, n_events=n_events), w)]
+    for k, nel in enumerate(configs.EXTRA_SECTIONS):
+        ws = configs.section_workload(nel, nt=nt, seed=k)
+        runs.append((configs.build(ws, seed=10 + k, nt=nt), ws))
+
+    def make(run, ww, n):
+        s = Solver.from_run(run, adjoint_flag=0, pml=ww.pml, nt=nt, n_events=n)
         s.set_model(run.model)
         for k in range(n):
             s.load_event(k, run.events[k])
         s.init()
-        s.run(1, 5)
-        s.synchronize()
-        t0 = time.perf_counter()
-        s.run(6, steps)
-        dt = time.perf_counter() - t0
-        res[n] = w.n_points * steps * n / dt
-        s.close()
-    return {"workload": f"forward {w.name} anelastic, {n_events} events (seeded sources 5-600 km deep, "
-                        f"10-400 receivers each)", "plain_updates_per_s": res[1],
-            "batched_updates_per_s": res[n_events], "batched_per_event_over_plain": res[n_events] / res[1],
-            "n_events": n_events, "steps": steps}
+        return s
+
+    # one event alone on the main mesh
+    s = make(runs[0][0], w, 1)
+    s.autotune(1, tune)
+    s.run(tune + 1, warm)
+    plain = w.n_points * steps / _timed_run(s, tune + warm + 1, steps)
+    s.close()
+    sols = [make(runs[0][0], w, n_events)] + [make(r, ww, 1) for r, ww in runs[1:]]
+    pts = w.n_points * n_events + sum(ww.n_points for _, ww in runs[1:])
+    # the same sections one context after another
+    seq = 0.0
+    for x in sols:
+        x.autotune(1, tune)
+        x.run(tune + 1, warm)
+        seq += _timed_run(x, tune + warm + 1, steps)
+    for x in sols:
+        x.init()
+    g = Group(sols)
+    sched = g.autotune(1, tune)
+    g.run(tune + 1, warm)
+    grouped = pts * steps / _timed_run(g, tune + warm + 1, steps)
+    ok = all(np.isfinite(x.seismograms(k)).all() for x in sols for k in range(x.n_events))
+    g.close()
+    for x in sols:
+        x.close()
+    return {"workload": (f"forward, {n_events} events on {w.name} (seeded sources 5-600 km deep, 10-400 "
+                         f"receivers each) + sections " + ", ".join("x".join(map(str, n)) for n in
+                                                                    configs.EXTRA_SECTIONS) + ", anelastic"),
+            "gll_points_per_step": pts, "steps": steps,
+            "plain_updates_per_s": plain, "batched_updates_per_s": grouped,
+            "batched_per_event_over_plain": grouped / plain,
+            "sequential_updates_per_s": pts * steps / seq, "group_over_sequential": grouped / (pts * steps / seq),
+            "schedule": {k: v for k, v in sched.items() if k != "trial_ms"}, "seismograms_finite": bool(ok)}
 
 
 def main():
@@ -226,7 +264,10 @@ def main():
     s.init()
     del run
     print(f"# initialised; device memory {s.memory_bytes() / 2**30:.2f} GiB", file=sys.stderr, flush=True)
-    if args.warmup:
+    sched = None
+    if args.warmup >= 6:
+        sched = s.autotune(1, args.warmup)    # bit-identical steps under trial launch schedules
+    elif args.warmup:
         s.run(1, args.warmup)
     s.set_profiling(True)
     torch.cuda.synchronize()
@@ -269,6 +310,7 @@ def main():
                      "traffic_frac": (traffic / (elem_ms * 1e-3) / HBM_PEAK) if traffic else None},
         "step_roofline_frac": value * ab["step"] / HBM_PEAK,
         "kernel_ms_per_step": {"element": elem_ms, "node": tm["node_ms"] / max(tm["n_element"], 1)},
+        "schedule": sched,
     }
     s.close()
     if not args.no_gradient:

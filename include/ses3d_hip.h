@@ -191,6 +191,44 @@ int ses3d_hip_set_profiling(ses3d_hip_context *ctx, int32_t on);
 int ses3d_hip_get_timing(ses3d_hip_context *ctx, double *total_ms, double *element_ms,
                          double *node_ms, int32_t *n_element_launches);
 
+/* Launch schedule (no counterpart in the reference, which has no launches):
+ * sched[0] elements per element-kernel workgroup (1, 2, 4), sched[1] lanes per
+ * node-kernel workgroup (128, 256, 512), sched[2] / sched[3] XCD-aware
+ * workgroup order of the element / node kernel (0, 1).  Every schedule gives
+ * bit-identical results.  ses3d_hip_autotune advances steps it_begin ..
+ * it_begin+n_steps-1 exactly as ses3d_hip_run would (n_steps >= 6), timing 6
+ * trial schedules over the first 6*(n_steps/6) of them, and keeps the fastest
+ * element and node schedules for later runs; get_schedule returns the current
+ * schedule and, when trial_ms is not NULL, the 6 trials' fastest element and
+ * node kernel times (12 floats, -1 before any autotune). */
+int ses3d_hip_set_schedule(ses3d_hip_context *ctx, const int32_t *sched);
+int ses3d_hip_get_schedule(ses3d_hip_context *ctx, int32_t *sched, float *trial_ms);
+int ses3d_hip_autotune(ses3d_hip_context *ctx, int32_t it_begin, int32_t n_steps);
+
+/* Groups: several initialised contexts (sections of any size, each with its
+ * own events, receivers and sources; all elastic or all anelastic; one
+ * adjoint_flag) advanced together by one element-kernel launch and one
+ * node-kernel launch per step, captured into hipGraphs like a single context.
+ * No counterpart in the reference, where each section is a separate MPI job
+ * (SOURCE/ses3d_main.f90:95-178 per event).  group_run(g, it_begin, n) gives
+ * every member exactly the results of ses3d_hip_run(member, it_begin, n);
+ * members keep their own outputs (seismograms, fields, kernels) and may still
+ * be run alone.  The group has its own stream and schedule; work issued on a
+ * member's stream after a group call is ordered after it.  Members must
+ * outlive the group. */
+typedef struct ses3d_hip_group ses3d_hip_group;
+int ses3d_hip_group_create(ses3d_hip_context *const *ctxs, int32_t n, ses3d_hip_group **out);
+void ses3d_hip_group_destroy(ses3d_hip_group *grp);
+int ses3d_hip_group_run(ses3d_hip_group *grp, int32_t it_begin, int32_t n_steps);
+int ses3d_hip_group_autotune(ses3d_hip_group *grp, int32_t it_begin, int32_t n_steps);
+int ses3d_hip_group_synchronize(ses3d_hip_group *grp);
+int ses3d_hip_group_set_schedule(ses3d_hip_group *grp, const int32_t *sched);
+int ses3d_hip_group_get_schedule(ses3d_hip_group *grp, int32_t *sched, float *trial_ms);
+int ses3d_hip_group_set_graph_steps(ses3d_hip_group *grp, int32_t steps);
+int ses3d_hip_group_set_profiling(ses3d_hip_group *grp, int32_t on);
+int ses3d_hip_group_get_timing(ses3d_hip_group *grp, double *total_ms, double *element_ms,
+                               double *node_ms, int32_t *n_element_launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,8 @@ Workloads (spherical section colat 40-60 deg, lon 10-30 deg, r 5371-6371 km):
                        dt=0.1, nt=4000, 200 receivers
   gradient_48x48x24    anelastic, true = 1D + 2% blobs, 100 receivers, dt=0.15, nt=3000
   batched_64x64x32     32 events, anelastic, 10-400 receivers each, nt=3000
+  + EXTRA_SECTIONS     six more anelastic sections of uneven element counts (one
+                       event each) advanced with the 32 events in one launch per step
 """
 from __future__ import annotations
 
@@ -46,6 +48,18 @@ WORKLOADS = {
     "batched_64x64x32": Workload("batched_64x64x32", (64, 64, 32), True, 2, 0.1, 3000, 200, "grf",
                                  n_events=32),
 }
+
+
+# the uneven sections batched with the 64x64x32 events (elements per axis)
+EXTRA_SECTIONS = [(16, 16, 16), (24, 24, 12), (32, 32, 16), (40, 40, 20), (48, 48, 24), (56, 56, 28)]
+
+
+def section_workload(nel, nt: int = 3000, seed: int = 0) -> Workload:
+    """One extra anelastic section (3D random model, dt=0.1, pml=2) with a
+    seeded receiver count in 10..400."""
+    rng = np.random.default_rng(1000 + seed)
+    return Workload(f"section_{nel[0]}x{nel[1]}x{nel[2]}", tuple(nel), True, 2, 0.1, nt,
+                    int(rng.integers(10, 401)), "grf")
 
 
 def _event(nt, dt, rng, colat, lon, depth, rec: F.Receivers, stf, name="1") -> I.EventInput:

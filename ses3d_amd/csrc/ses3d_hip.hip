@@ -54,10 +54,17 @@ int fail(int code, const std::string &msg) {
 #ifndef SES3D_WAVES_PER_EU
 #define SES3D_WAVES_PER_EU 0
 #endif
-constexpr int EPB = SES3D_EPB;   // elements per block, element kernel
-constexpr int TPE = 128;    // threads per element (125 active)
-constexpr int ELEM_THREADS = EPB * TPE;
-constexpr int NODE_THREADS = 256;   // 4 node blocks of 64 per workgroup
+// Launch schedule: elements per element-kernel workgroup (1, 2 or 4; 128
+// lanes each, 125 active), lanes per node-kernel workgroup (128, 256 or 512:
+// 2, 4 or 8 node blocks of 64) and the XCD-aware workgroup order.  Every
+// schedule computes bit-identical results; the autotuner picks by timing.
+constexpr int DEFAULT_EPB = SES3D_EPB;
+constexpr int TPE = 128;                 // threads per element (125 active)
+constexpr int DEFAULT_NODE_THREADS = 256;
+#ifndef SES3D_NODE_MAX_THREADS
+#define SES3D_NODE_MAX_THREADS 512
+#endif
+constexpr int NODE_MAX_THREADS = SES3D_NODE_MAX_THREADS;
 
 // component slots (same order for e, d, split stresses)
 enum { EXX = 0, EYY, EZZ, EXY, EYX, EXZ, EZX, EYZ, EZY };
@@ -171,20 +178,15 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // forward mode (MODE 0) the displacement gradient feeds nothing but the
 // absorbing terms, so it is integrated only inside the layers.
 // ---------------------------------------------------------------------------
-template <int DISS, int MODE, int ANISO>
-__global__ __launch_bounds__(ELEM_THREADS)
-#if SES3D_WAVES_PER_EU
-__attribute__((amdgpu_waves_per_eu(SES3D_WAVES_PER_EU, 8)))
-#endif
-void k_element(const MeshDev *__restrict__ mesh,
-                                                          const EventDev *__restrict__ evs, const int *itp,
-                                                          int blocks_per_event) {
-    const MeshDev &Ms = *mesh;
-    const int it = *itp;   // the step number lives in HBM so a captured step graph replays unchanged
-    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+// One element-kernel workgroup: EPB elements of one event.  `blk` is the
+// workgroup's index among the section's blocks_per_event * n_events tiles.
+// MT / ET: MeshDev / EventDev, plain or in the constant address space.
+template <int DISS, int MODE, int ANISO, int EPB, typename MT, typename ET>
+__device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const int it, const int blk,
+                                             const int blocks_per_event) {
     const int ev = blk / blocks_per_event;
     const int eb = blk - ev * blocks_per_event;
-    const EventDev &E = evs[ev];
+    auto &E = evs[ev];
     const int t = threadIdx.x;
     const int sub = t >> 7, nd = t & (TPE - 1);
     const int NE = (int)Ms.NE;
@@ -195,7 +197,7 @@ void k_element(const MeshDev *__restrict__ mesh,
     __shared__ float tabs[7 * 25];
     __shared__ float sv[EPB][3][TPE];
     __shared__ float ss[EPB][9][TPE];
-    for (int k = t; k < 7 * 25; k += ELEM_THREADS) tabs[k] = ldg(Ms.tab + k);
+    for (int k = t; k < 7 * 25; k += EPB * TPE) tabs[k] = Ms.tab[k];
     const float *D1 = tabs, *D2 = tabs + 25, *D3 = tabs + 50, *WDL = tabs + 75;
     const float *DUM1 = tabs + 100, *DUM2 = tabs + 125, *DUM3 = tabs + 150;
 
@@ -508,6 +510,56 @@ void k_element(const MeshDev *__restrict__ mesh,
     }
 }
 
+// One section: the step number lives in HBM so a captured step graph replays unchanged.
+template <int DISS, int MODE, int ANISO, int EPB>
+__global__ __launch_bounds__(EPB * TPE)
+#if SES3D_WAVES_PER_EU
+__attribute__((amdgpu_waves_per_eu(SES3D_WAVES_PER_EU, 8)))
+#endif
+void k_element(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs, const int *itp,
+               int blocks_per_event, int remap) {
+    const int blk = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    element_step<DISS, MODE, ANISO, EPB>(*mesh, evs, *itp, blk, blocks_per_event);
+}
+
+// Several sections (meshes of any size, each with its own events and step
+// counter) advanced by one launch: the workgroups of section s are
+// [eb0, eb0 + bpe * n_events) for the element kernel and
+// [nb0, nb0 + ceil(nbpe * 64 * n_events / lanes)) for the node kernel.
+struct SectionDev {
+    const MeshDev *mesh;
+    const EventDev *evs;
+    int *itp;
+    int n_events;
+    int eb0, bpe;     // element kernel: first workgroup, workgroups per event
+    int nb0, nbpe;    // node kernel: first workgroup, 4x4x4 node blocks per event
+};
+
+// Launch tables are read-only while a kernel runs: reading them through the
+// constant address space makes their uniform reads scalar loads (through a
+// pointer loaded from memory they would be per-lane flat loads).
+typedef __attribute__((address_space(4))) const SectionDev cSectionDev;
+typedef __attribute__((address_space(4))) const MeshDev cMeshDev;
+typedef __attribute__((address_space(4))) const EventDev cEventDev;
+
+__device__ __forceinline__ int section_of(cSectionDev *secs, int nsec, int b, bool node) {
+    int s = 0;
+    while (s + 1 < nsec && b >= (node ? secs[s + 1].nb0 : secs[s + 1].eb0)) ++s;
+    return s;
+}
+
+template <int DISS, int MODE, int ANISO, int EPB>
+__global__ __launch_bounds__(EPB * TPE)
+#if SES3D_WAVES_PER_EU
+__attribute__((amdgpu_waves_per_eu(SES3D_WAVES_PER_EU, 8)))
+#endif
+void k_element_group(const SectionDev *__restrict__ secs, int nsec, int remap) {
+    const int b = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    cSectionDev *cs = (cSectionDev *)secs;
+    cSectionDev &S = cs[section_of(cs, nsec, b, false)];
+    element_step<DISS, MODE, ANISO, EPB>(*(cMeshDev *)S.mesh, (cEventDev *)S.evs, *S.itp, b - S.eb0, S.bpe);
+}
+
 // ---------------------------------------------------------------------------
 // point forces (forward source_type 1/2/3, adjoint sources): misc.f90:29-92.
 // One block per (event, source element); each thread one node; the element's
@@ -551,14 +603,11 @@ __global__ void k_forces(const MeshDev *__restrict__ mesh, const EventDev *__res
 // (:326-328), taper (:366).  One thread per global node; a wavefront owns one
 // 4x4x4 node block.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict__ mesh,
-                                                       const EventDev *__restrict__ evs,
-                                                       int nodeblocks_per_event, int n_events, int *itp) {
-    const MeshDev &Ms = *mesh;
-    // last kernel of a step: advance the step number (no kernel of this launch reads it)
-    if (blockIdx.x == 0 && threadIdx.x == 0) *itp = *itp + 1;
-    // 32-bit index math: n_events * padded nodes stays below 2^31 (checked on the host)
-    const int gt = xcd_remap(blockIdx.x, gridDim.x) * NODE_THREADS + threadIdx.x;
+// `gt`: the lane's index among the section's n_events * nodeblocks_per_event * 64
+// node lanes (32-bit: n_events * padded nodes stays below 2^31, checked on the host).
+template <typename MT, typename ET>
+__device__ __forceinline__ void node_update(MT &Ms, ET *__restrict__ evs, const int nodeblocks_per_event,
+                                            const int n_events, const int gt) {
     const int nbt = gt >> 6;
     const int ev = nbt / nodeblocks_per_event;
     if (ev >= n_events) return;
@@ -569,25 +618,23 @@ __global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict
     const int bx = nb - byz * NBX, bz = byz / NBY, by = byz - bz * NBY;
     const int a = loc & 3, b = (loc >> 2) & 3, c = loc >> 4;
     if ((bx == NX && a) || (by == NY && b) || (bz == NZ && c)) return;
-    const EventDev &E = evs[ev];
+    auto &E = evs[ev];
     const size_t NP = (size_t)Ms.NP;
-    // sharers per axis, ascending element index
-    int xe[2], xl[2], nxs = 0, ye[2], yl[2], nys = 0, ze[2], zl[2], nzs = 0;
-    if (a) { xe[0] = bx; xl[0] = a; nxs = 1; }
-    else {
-        if (bx > 0) { xe[nxs] = bx - 1; xl[nxs] = 4; ++nxs; }
-        if (bx < NX) { xe[nxs] = bx; xl[nxs] = 0; ++nxs; }
-    }
-    if (b) { ye[0] = by; yl[0] = b; nys = 1; }
-    else {
-        if (by > 0) { ye[nys] = by - 1; yl[nys] = 4; ++nys; }
-        if (by < NY) { ye[nys] = by; yl[nys] = 0; ++nys; }
-    }
-    if (c) { ze[0] = bz; zl[0] = c; nzs = 1; }
-    else {
-        if (bz > 0) { ze[nzs] = bz - 1; zl[nzs] = 4; ++nzs; }
-        if (bz < NZ) { ze[nzs] = bz; zl[nzs] = 0; ++nzs; }
-    }
+    // sharers per axis in ascending element index: (e0, l0) and, on an
+    // element face, (e1, l1); held in scalars so nothing goes to scratch
+    int xe0, xl0, nxs;
+    if (a) { xe0 = bx; xl0 = a; nxs = 1; }
+    else if (bx > 0) { xe0 = bx - 1; xl0 = 4; nxs = bx < NX ? 2 : 1; }
+    else { xe0 = bx; xl0 = 0; nxs = 1; }
+    int ye0, yl0, nys;
+    if (b) { ye0 = by; yl0 = b; nys = 1; }
+    else if (by > 0) { ye0 = by - 1; yl0 = 4; nys = by < NY ? 2 : 1; }
+    else { ye0 = by; yl0 = 0; nys = 1; }
+    int ze0, zl0, nzs;
+    if (c) { ze0 = bz; zl0 = c; nzs = 1; }
+    else if (bz > 0) { ze0 = bz - 1; zl0 = 4; nzs = bz < NZ ? 2 : 1; }
+    else { ze0 = bz; zl0 = 0; nzs = 1; }
+    // a second sharer exists only on a face between elements: it is (bx, 0) (likewise for y, z)
     // load the <=8 contributions first (independent loads), then add them in
     // the reference's order: sharers sorted by (i, j, k), i slowest
     float fx[8], fy[8], fz[8];
@@ -600,7 +647,10 @@ __global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict
                 const int sl = ii * 4 + jj * 2 + kk;
                 fx[sl] = fy[sl] = fz[sl] = 0.f;
                 if (ii < nxs && jj < nys && kk < nzs) {
-                    const int q = (xe[ii] + NX * (ye[jj] + NY * ze[kk])) * 125 + (xl[ii] + 5 * (yl[jj] + 5 * zl[kk]));
+                    const int ex = ii ? bx : xe0, lx = ii ? 0 : xl0;
+                    const int ey = jj ? by : ye0, ly = jj ? 0 : yl0;
+                    const int ez = kk ? bz : ze0, lz = kk ? 0 : zl0;
+                    const int q = (ex + NX * (ey + NY * ez)) * 125 + (lx + 5 * (ly + 5 * lz));
                     fx[sl] = ldnt(E.f + q);
                     fy[sl] = ldnt(E.f + NP + q);
                     fz[sl] = ldnt(E.f + 2 * NP + q);
@@ -645,6 +695,26 @@ __global__ __launch_bounds__(NODE_THREADS) void k_node(const MeshDev *__restrict
     stnt(v + 2 * Ms.NGB + g, tg * vz);
 }
 
+__global__ __launch_bounds__(NODE_MAX_THREADS) void k_node(const MeshDev *__restrict__ mesh,
+                                                           const EventDev *__restrict__ evs,
+                                                           int nodeblocks_per_event, int n_events, int *itp,
+                                                           int remap) {
+    // last kernel of a step: advance the step number (no kernel of this launch reads it)
+    if (blockIdx.x == 0 && threadIdx.x == 0) *itp = *itp + 1;
+    const int b = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    node_update(*mesh, evs, nodeblocks_per_event, n_events, b * (int)blockDim.x + (int)threadIdx.x);
+}
+
+__global__ __launch_bounds__(NODE_MAX_THREADS) void k_node_group(const SectionDev *__restrict__ secs, int nsec,
+                                                                 int remap) {
+    const int b = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    cSectionDev *cs = (cSectionDev *)secs;
+    cSectionDev &S = cs[section_of(cs, nsec, b, true)];
+    if (b == S.nb0 && threadIdx.x == 0) *S.itp = *S.itp + 1;
+    node_update(*(cMeshDev *)S.mesh, (cEventDev *)S.evs, S.nbpe, S.n_events,
+                (b - S.nb0) * (int)blockDim.x + (int)threadIdx.x);
+}
+
 // Count, for each divisor b[d], the float32 numerators a (all 2^32 bit
 // patterns except NaN/Inf) for which div_recip(a, b, y) differs from a / b.
 __global__ void k_verify_div(const float *b, const float *y, int nd, unsigned long long *bad) {
@@ -668,9 +738,9 @@ __global__ void k_set_step(int *itp, int it) {
 }
 
 // mass matrix (SOURCE/ses3d_init.f90:254-277): the same ordered gather as k_node
-__global__ __launch_bounds__(NODE_THREADS) void k_mass(MeshDev *mesh, float *MM, const float *w5) {
+__global__ __launch_bounds__(DEFAULT_NODE_THREADS) void k_mass(MeshDev *mesh, float *MM, const float *w5) {
     const MeshDev &Ms = *mesh;
-    const long long gt = (long long)blockIdx.x * NODE_THREADS + threadIdx.x;
+    const long long gt = (long long)blockIdx.x * DEFAULT_NODE_THREADS + threadIdx.x;
     const long long nb = gt >> 6;
     const int loc = (int)(gt & 63);
     const int NX = Ms.NX, NY = Ms.NY, NZ = Ms.NZ, NBX = Ms.NBX, NBY = Ms.NBY;
@@ -753,6 +823,53 @@ struct GraphEntry {
     hipGraphExec_t exec = nullptr;
 };
 
+// Launch schedule of the element and node kernels.
+struct Schedule {
+    int epb = DEFAULT_EPB;                    // elements per element-kernel workgroup: 1, 2, 4
+    int node_threads = DEFAULT_NODE_THREADS;  // lanes per node-kernel workgroup: 128, 256, 512
+    int remap_elem = 1, remap_node = 0;       // XCD-aware workgroup order (measured defaults)
+};
+
+constexpr int N_TRIALS = 6;
+
+// The time-step driver shared by one section (ses3d_hip_context) and a group
+// of sections advanced together (ses3d_hip_group): stream-ordered steps,
+// capture and replay of step graphs, per-kernel timing and the autotuner.
+struct StepRunner {
+    hipStream_t stream = nullptr;
+    Schedule sched;
+    int it_host = -1;                 // value the device step counter(s) hold next (-1 unknown)
+    int graph_steps = 16;
+    std::vector<GraphEntry> graphs;
+    bool profiling = false;
+    double t_total = 0, t_elem = 0, t_node = 0;
+    int n_elem = 0;
+    std::vector<hipEvent_t> tev;
+    float tune_ms[N_TRIALS][2] = {};  // autotuner: fastest element / node kernel per trial
+    bool tuned = false;
+
+    virtual ~StepRunner() {}
+    // the kernels of one step for every event, reading the step number from HBM
+    virtual int launch_receivers(bool record) = 0;
+    virtual int launch_element(int mode) = 0;
+    virtual int launch_forces() = 0;
+    virtual int launch_node() = 0;
+    virtual int set_step(int it) = 0;   // store it in every step counter, stream-ordered
+    virtual int schedule_changed() {
+        drop_graphs();
+        return 0;
+    }
+    void drop_graphs() {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
+    void release_runner() {
+        drop_graphs();
+        for (auto &e : tev) (void)hipEventDestroy(e);
+        tev.clear();
+    }
+};
+
 struct EventHost {
     int source_type = 0;
     float xxs = 0, yys = 0, zzs = 0, mom[6] = {0, 0, 0, 0, 0, 0};
@@ -769,6 +886,7 @@ struct EventHost {
     float *seis = nullptr;     // nr x 3 x nt
     float *rec_w = nullptr;    // nr x 125
     float *force_w = nullptr, *force_stf = nullptr;
+    size_t seis_cap = 0, rec_w_cap = 0, force_w_cap = 0, force_stf_cap = 0;
     double *grad = nullptr;
     float *hist = nullptr;       // sampled forward fields the kernels read/write (active buffer)
     float *hist_own = nullptr;   // full history of every sampled step (allocated on first use)
@@ -782,13 +900,13 @@ struct EventHost {
     int seg_slots = 0;
 };
 
-struct ses3d_hip_context {
+struct ses3d_hip_context : StepRunner {
     ses3d_hip_config cfg;
     int n_events = 0;
     int NX = 0, NY = 0, NZ = 0, NBX = 0, NBY = 0, NBZ = 0;
     long long NE = 0, NP = 0, NGB = 0;
-    hipStream_t stream = nullptr;
     int device = 0;
+    int generation = 0;               // counts inits (kernel arguments may change)
     ses3d::MeshSetup ms;
     bool model_set = false, initialised = false;
     // device mesh data
@@ -809,23 +927,64 @@ struct ses3d_hip_context {
     ForceGroup *groups_d = nullptr;
     ForceDev *forces_d = nullptr;
     int n_groups = 0;
+    size_t recs_cap = 0, groups_cap = 0, forces_cap = 0;
     std::vector<int> saving_vector;   // 1..nt
     int ckpt_k = 0;
     bool ckpt_ready = false;
-    // step number in HBM, and captured step graphs
+    // step number in HBM
     int *it_d = nullptr;
-    int it_host = -1;                 // value the device counter holds next (-1 unknown)
-    int graph_steps = 16;
     unsigned long long div_bad[4] = {0, 0, 0, 0};   // numerators failing the reciprocal check
-    std::vector<GraphEntry> graphs;
-    std::vector<void *> allocs;
+    std::vector<std::pair<void *, size_t>> allocs;
     int64_t bytes = 0;
-    // timing
-    bool profiling = false;
-    double t_total = 0, t_elem = 0, t_node = 0;
-    int n_elem = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<hipEvent_t> tev;
+    unsigned long long *div_buf = nullptr;        // divisors, reciprocals, failure counts
+
+    int launch_receivers_on(hipStream_t st, bool record);
+    int launch_forces_on(hipStream_t st);
+    int set_step_on(hipStream_t st, int it);
+    int launch_receivers(bool record) override { return launch_receivers_on(stream, record); }
+    int launch_element(int mode) override;
+    int launch_forces() override { return launch_forces_on(stream); }
+    int launch_node() override;
+    int set_step(int it) override { return set_step_on(stream, it); }
+};
+
+// Sections of any size (each a context with its own events) advanced by one
+// element-kernel and one node-kernel launch per step.
+struct ses3d_hip_group : StepRunner {
+    std::vector<ses3d_hip_context *> ctx;
+    std::vector<int> gens;            // member generations the launch table was built for
+    SectionDev *secs_d = nullptr;
+    int elem_wgs = 0, node_wgs = 0;   // workgroups of one group launch
+    int diss = 0, aniso = 0, af = 0;
+
+    int build_table();
+    int schedule_changed() override {
+        drop_graphs();
+        return build_table();
+    }
+    int launch_receivers(bool record) override {
+        for (auto *c : ctx) {
+            int rc = c->launch_receivers_on(stream, record);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    int launch_element(int mode) override;
+    int launch_forces() override {
+        for (auto *c : ctx) {
+            int rc = c->launch_forces_on(stream);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    int launch_node() override;
+    int set_step(int it) override {
+        for (auto *c : ctx) {
+            int rc = c->set_step_on(stream, it);
+            if (rc) return rc;
+        }
+        return 0;
+    }
 };
 
 namespace {
@@ -837,10 +996,20 @@ int dalloc(ses3d_hip_context *c, T **p, size_t n) {
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess)
         return fail(SES3D_HIP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
-    c->allocs.push_back(q);
+    c->allocs.push_back({q, bytes});
     c->bytes += (int64_t)bytes;
     *p = (T *)q;
     return 0;
+}
+
+void dfree(ses3d_hip_context *c, void *p) {
+    for (size_t k = 0; k < c->allocs.size(); ++k)
+        if (c->allocs[k].first == p) {
+            c->bytes -= (int64_t)c->allocs[k].second;
+            c->allocs.erase(c->allocs.begin() + (long)k);
+            (void)hipFree(p);
+            return;
+        }
 }
 
 #define TRY(expr)                       \
@@ -848,6 +1017,22 @@ int dalloc(ses3d_hip_context *c, T **p, size_t n) {
         int _rc = (expr);               \
         if (_rc) return _rc;            \
     } while (0)
+
+// (re)allocate *p for n elements unless it already holds that many (inits
+// after new receiver or source lists may need larger buffers)
+template <typename T>
+int dfit(ses3d_hip_context *c, T **p, size_t &cap, size_t n) {
+    if (*p && cap >= n) return 0;
+    if (*p) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        dfree(c, *p);
+        *p = nullptr;
+        cap = 0;
+    }
+    TRY(dalloc(c, p, n));
+    cap = n;
+    return 0;
+}
 
 // reference layout (element fastest, node slowest) <-> element-major layout
 void ref_to_em(const ses3d_hip_context *c, const float *src, float *dst) {
@@ -891,141 +1076,329 @@ int zero(ses3d_hip_context *c, void *p, size_t bytes) {
     return 0;
 }
 
-int elem_blocks(const ses3d_hip_context *c) { return (int)((c->NE + EPB - 1) / EPB); }
+int elem_blocks(const ses3d_hip_context *c, int epb) { return (int)((c->NE + epb - 1) / epb); }
 long long node_blocks(const ses3d_hip_context *c) { return (long long)c->NBX * c->NBY * c->NBZ; }
 
-int launch_element(ses3d_hip_context *c, int mode) {
-    const int bpe = elem_blocks(c);
-    dim3 grid((unsigned)(bpe * c->n_events)), block(ELEM_THREADS);
-    const int key = (c->cfg.is_diss ? 100 : 0) + mode * 10 + (c->aniso ? 1 : 0);
-#define LAUNCH_E(D, MO, AN)                                                                                   \
-    case D * 100 + MO * 10 + AN:                                                                              \
-        hipLaunchKernelGGL((k_element<D, MO, AN>), grid, block, 0, c->stream, c->mesh_d, c->ev_d, c->it_d, bpe); \
-        break;
-    switch (key) {
-        LAUNCH_E(0, 0, 0) LAUNCH_E(0, 0, 1) LAUNCH_E(0, 1, 0) LAUNCH_E(0, 1, 1) LAUNCH_E(0, 2, 0) LAUNCH_E(0, 2, 1)
-        LAUNCH_E(1, 0, 0) LAUNCH_E(1, 0, 1) LAUNCH_E(1, 1, 0) LAUNCH_E(1, 1, 1) LAUNCH_E(1, 2, 0) LAUNCH_E(1, 2, 1)
+// One element-kernel launch: a single section (secs == nullptr) or a group.
+struct ElemLaunch {
+    hipStream_t stream;
+    int epb, remap;
+    unsigned wgs;
+    const MeshDev *mesh;
+    const EventDev *evs;
+    const int *itp;
+    int bpe;
+    const SectionDev *secs;
+    int nsec;
+};
+
+template <int D, int MO, int AN, int EP>
+void launch_elem_epb(const ElemLaunch &L) {
+    if (L.secs)
+        hipLaunchKernelGGL((k_element_group<D, MO, AN, EP>), dim3(L.wgs), dim3(EP * TPE), 0, L.stream, L.secs, L.nsec,
+                           L.remap);
+    else
+        hipLaunchKernelGGL((k_element<D, MO, AN, EP>), dim3(L.wgs), dim3(EP * TPE), 0, L.stream, L.mesh, L.evs, L.itp,
+                           L.bpe, L.remap);
+}
+
+template <int D, int MO, int AN>
+int launch_elem_variant(const ElemLaunch &L) {
+    switch (L.epb) {
+        case 1: launch_elem_epb<D, MO, AN, 1>(L); break;
+        case 2: launch_elem_epb<D, MO, AN, 2>(L); break;
+        case 4: launch_elem_epb<D, MO, AN, 4>(L); break;
+        default: return fail(SES3D_HIP_EINVAL, "elements per workgroup must be 1, 2 or 4");
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_elem(int diss, int mode, int aniso, const ElemLaunch &L) {
+    switch ((diss ? 100 : 0) + mode * 10 + (aniso ? 1 : 0)) {
+        case 0: return launch_elem_variant<0, 0, 0>(L);
+        case 1: return launch_elem_variant<0, 0, 1>(L);
+        case 10: return launch_elem_variant<0, 1, 0>(L);
+        case 11: return launch_elem_variant<0, 1, 1>(L);
+        case 20: return launch_elem_variant<0, 2, 0>(L);
+        case 21: return launch_elem_variant<0, 2, 1>(L);
+        case 100: return launch_elem_variant<1, 0, 0>(L);
+        case 101: return launch_elem_variant<1, 0, 1>(L);
+        case 110: return launch_elem_variant<1, 1, 0>(L);
+        case 111: return launch_elem_variant<1, 1, 1>(L);
+        case 120: return launch_elem_variant<1, 2, 0>(L);
+        case 121: return launch_elem_variant<1, 2, 1>(L);
         default: return fail(SES3D_HIP_EINVAL, "no element kernel for this mode");
     }
-#undef LAUNCH_E
-    HIP_TRY(hipGetLastError());
-    return 0;
 }
 
-int launch_node(ses3d_hip_context *c) {
-    const long long nb = node_blocks(c);
-    const long long threads = nb * 64 * c->n_events;
-    dim3 grid((unsigned)((threads + NODE_THREADS - 1) / NODE_THREADS)), block(NODE_THREADS);
-    hipLaunchKernelGGL(k_node, grid, block, 0, c->stream, c->mesh_d, c->ev_d, (int)nb, c->n_events, c->it_d);
-    HIP_TRY(hipGetLastError());
-    return 0;
+bool valid_schedule(const Schedule &s) {
+    return (s.epb == 1 || s.epb == 2 || s.epb == 4) &&
+           (s.node_threads == 128 || s.node_threads == 256 || s.node_threads == 512) &&
+           (s.remap_elem == 0 || s.remap_elem == 1) && (s.remap_node == 0 || s.remap_node == 1);
 }
 
-// One iteration of the reference time loop for all events.  With profiling on,
-// HIP events bracket the element and node kernels on the context's stream;
-// they are read back after the run, so the stream is never stalled.
-int launch_step(ses3d_hip_context *c, int mode, bool record, bool timed);
-
-int step(ses3d_hip_context *c, int it, int mode, bool record) {
-    const int nt = c->cfg.nt;
-    if (c->it_host != it) {
-        hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, c->stream, c->it_d, it);
-        HIP_TRY(hipGetLastError());
-    }
-    TRY(launch_step(c, mode, record, c->profiling));
-    c->it_host = it + 1;
-    (void)nt;
-    return 0;
+// The autotuner's trials: trial k times element schedule (epb, order) k and
+// node schedule (lanes, order) k in the same steps; the two kernels are timed
+// separately and chosen independently.
+Schedule trial_schedule(int k) {
+    static const int epb[3] = {1, 2, 4}, lanes[3] = {128, 256, 512};
+    Schedule s;
+    s.epb = epb[k % 3];
+    s.node_threads = lanes[k % 3];
+    s.remap_elem = s.remap_node = k / 3;
+    return s;
 }
 
-// The kernels of one step, reading the step number from HBM (graph-capturable).
-int launch_step(ses3d_hip_context *c, int mode, bool record, bool timed) {
-    const int nt = c->cfg.nt;
-    if (record && c->nrec_total > 0) {
-        const int threads = 3 * c->nrec_total;
-        hipLaunchKernelGGL(k_receivers, dim3((threads + 127) / 128), dim3(128), 0, c->stream, c->mesh_d,
-                           c->ev_d, c->recs_d, c->nrec_total, nt, c->it_d);
-        HIP_TRY(hipGetLastError());
-    }
+// One iteration of the reference time loop for all events, reading the step
+// number from HBM (graph-capturable).  With `timed`, HIP events bracket the
+// element and node kernels on the runner's stream; they are read back after
+// the run, so the stream is never stalled.
+int launch_step(StepRunner *r, int mode, bool record, bool timed) {
+    TRY(r->launch_receivers(record));
     hipEvent_t *ev = nullptr;
     if (timed) {
-        const size_t need = 4 * (size_t)(c->n_elem + 1);
-        while (c->tev.size() < need) {
+        const size_t need = 4 * (size_t)(r->n_elem + 1);
+        while (r->tev.size() < need) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
-            c->tev.push_back(e);
+            r->tev.push_back(e);
         }
-        ev = &c->tev[4 * (size_t)c->n_elem];
-        HIP_TRY(hipEventRecord(ev[0], c->stream));
+        ev = &r->tev[4 * (size_t)r->n_elem];
+        HIP_TRY(hipEventRecord(ev[0], r->stream));
     }
-    TRY(launch_element(c, mode));
-    if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
-    if (c->n_groups > 0) {
-        hipLaunchKernelGGL(k_forces, dim3(c->n_groups), dim3(128), 0, c->stream, c->mesh_d, c->ev_d,
-                           c->groups_d, c->forces_d, nt, c->it_d);
-        HIP_TRY(hipGetLastError());
-    }
-    if (ev) HIP_TRY(hipEventRecord(ev[2], c->stream));
-    TRY(launch_node(c));
+    TRY(r->launch_element(mode));
+    if (ev) HIP_TRY(hipEventRecord(ev[1], r->stream));
+    TRY(r->launch_forces());
+    if (ev) HIP_TRY(hipEventRecord(ev[2], r->stream));
+    TRY(r->launch_node());
     if (ev) {
-        HIP_TRY(hipEventRecord(ev[3], c->stream));
-        c->n_elem += 1;
+        HIP_TRY(hipEventRecord(ev[3], r->stream));
+        r->n_elem += 1;
     }
+    return 0;
+}
+
+int step(StepRunner *r, int it, int mode, bool record, bool timed) {
+    if (r->it_host != it) TRY(r->set_step(it));
+    TRY(launch_step(r, mode, record, timed));
+    r->it_host = it + 1;
     return 0;
 }
 
 // Steps it_begin.. through graphs of graph_steps captured steps each (one graph
 // per (mode, record) pair, captured on first use), the tail eagerly.
-int run_steps(ses3d_hip_context *c, int it_begin, int n, int mode, bool record) {
+int run_steps(StepRunner *r, int it_begin, int n, int mode, bool record) {
     int it = it_begin;
-    const int G = c->graph_steps;
-    if (G > 0 && !c->profiling && n >= G) {
+    const int G = r->graph_steps;
+    if (G > 0 && !r->profiling && n >= G) {
         GraphEntry *ge = nullptr;
-        for (auto &g : c->graphs)
+        for (auto &g : r->graphs)
             if (g.mode == mode && g.record == record && g.steps == G) ge = &g;
         if (!ge) {
             GraphEntry g;
             g.mode = mode; g.record = record; g.steps = G;
             hipGraph_t graph;
-            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+            HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeRelaxed));
             int rc = 0;
-            for (int k = 0; k < G && !rc; ++k) rc = launch_step(c, mode, record, false);
-            hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            for (int k = 0; k < G && !rc; ++k) rc = launch_step(r, mode, record, false);
+            hipError_t e = hipStreamEndCapture(r->stream, &graph);
             if (rc) return rc;
             if (e != hipSuccess) return fail(SES3D_HIP_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
             HIP_TRY(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
             HIP_TRY(hipGraphDestroy(graph));
-            c->graphs.push_back(g);
-            ge = &c->graphs.back();
+            r->graphs.push_back(g);
+            ge = &r->graphs.back();
         }
-        if (c->it_host != it) {
-            hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, c->stream, c->it_d, it);
-            HIP_TRY(hipGetLastError());
-        }
+        if (r->it_host != it) TRY(r->set_step(it));
         while (it + G <= it_begin + n) {
-            HIP_TRY(hipGraphLaunch(ge->exec, c->stream));
+            HIP_TRY(hipGraphLaunch(ge->exec, r->stream));
             it += G;
         }
-        c->it_host = it;
+        r->it_host = it;
     }
-    for (; it < it_begin + n; ++it) TRY(step(c, it, mode, record));
+    for (; it < it_begin + n; ++it) TRY(step(r, it, mode, record, r->profiling));
     return 0;
 }
 
-int collect_timing(ses3d_hip_context *c) {
-    if (!c->profiling) return 0;
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    for (int k = 0; k < c->n_elem; ++k) {
+int collect_timing(StepRunner *r) {
+    if (!r->profiling) return 0;
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    for (int k = 0; k < r->n_elem; ++k) {
         float a = 0, b = 0;
-        HIP_TRY(hipEventElapsedTime(&a, c->tev[4 * k], c->tev[4 * k + 1]));
-        HIP_TRY(hipEventElapsedTime(&b, c->tev[4 * k + 2], c->tev[4 * k + 3]));
-        c->t_elem += a;
-        c->t_node += b;
+        HIP_TRY(hipEventElapsedTime(&a, r->tev[4 * k], r->tev[4 * k + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, r->tev[4 * k + 2], r->tev[4 * k + 3]));
+        r->t_elem += a;
+        r->t_node += b;
     }
     return 0;
 }
+
+// Steps it_begin..it_begin+n-1 as run_steps would take them, the first
+// N_TRIALS * (n / N_TRIALS) of them eagerly under the trial schedules (every
+// schedule gives bit-identical results), the rest under the fastest element
+// and node schedules found (the minimum kernel time over each trial's steps).
+int autotune_steps(StepRunner *r, int it_begin, int n, int mode, bool record) {
+    const int per = n / N_TRIALS;
+    if (per < 1) return fail(SES3D_HIP_EINVAL, "autotune needs at least 6 steps");
+    const Schedule keep = r->sched;
+    int it = it_begin;
+    r->n_elem = 0;
+    for (int k = 0; k < N_TRIALS; ++k) {
+        r->sched = trial_schedule(k);
+        TRY(r->schedule_changed());
+        for (int j = 0; j < per; ++j, ++it) TRY(step(r, it, mode, record, true));
+    }
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    int be = 0, bn = 0;
+    for (int k = 0; k < N_TRIALS; ++k) {
+        float te = 1e30f, tn = 1e30f;
+        for (int j = 0; j < per; ++j) {
+            const size_t q = 4 * (size_t)(k * per + j);
+            float a = 0, b = 0;
+            HIP_TRY(hipEventElapsedTime(&a, r->tev[q], r->tev[q + 1]));
+            HIP_TRY(hipEventElapsedTime(&b, r->tev[q + 2], r->tev[q + 3]));
+            te = std::min(te, a);
+            tn = std::min(tn, b);
+        }
+        r->tune_ms[k][0] = te;
+        r->tune_ms[k][1] = tn;
+        if (te < r->tune_ms[be][0]) be = k;
+        if (tn < r->tune_ms[bn][1]) bn = k;
+    }
+    Schedule s = keep;
+    const Schedule se = trial_schedule(be), sn = trial_schedule(bn);
+    s.epb = se.epb; s.remap_elem = se.remap_elem;
+    s.node_threads = sn.node_threads; s.remap_node = sn.remap_node;
+    r->sched = s;
+    TRY(r->schedule_changed());
+    r->tuned = true;
+    r->n_elem = 0;
+    return run_steps(r, it, it_begin + n - it, mode, record);
+}
+
+}  // namespace
+
+int ses3d_hip_context::launch_receivers_on(hipStream_t st, bool record) {
+    if (!(record && nrec_total > 0)) return 0;
+    const int threads = 3 * nrec_total;
+    hipLaunchKernelGGL(k_receivers, dim3((threads + 127) / 128), dim3(128), 0, st, mesh_d, ev_d, recs_d, nrec_total,
+                       cfg.nt, it_d);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ses3d_hip_context::launch_forces_on(hipStream_t st) {
+    if (n_groups <= 0) return 0;
+    hipLaunchKernelGGL(k_forces, dim3(n_groups), dim3(128), 0, st, mesh_d, ev_d, groups_d, forces_d, cfg.nt, it_d);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ses3d_hip_context::set_step_on(hipStream_t st, int it) {
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, st, it_d, it);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int ses3d_hip_context::launch_element(int mode) {
+    const int bpe = elem_blocks(this, sched.epb);
+    ElemLaunch L{stream, sched.epb, sched.remap_elem, (unsigned)(bpe * n_events), mesh_d, ev_d, it_d, bpe, nullptr, 0};
+    return launch_elem(cfg.is_diss, mode, aniso ? 1 : 0, L);
+}
+
+int ses3d_hip_context::launch_node() {
+    const long long threads = node_blocks(this) * 64 * n_events;
+    const int lanes = sched.node_threads;
+    hipLaunchKernelGGL(k_node, dim3((unsigned)((threads + lanes - 1) / lanes)), dim3(lanes), 0, stream, mesh_d, ev_d,
+                       (int)node_blocks(this), n_events, it_d, sched.remap_node);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Launch table of a group for its current schedule: sections in member order,
+// each a contiguous range of workgroups.
+int ses3d_hip_group::build_table() {
+    std::vector<SectionDev> h(ctx.size());
+    long long eb = 0, nb = 0;
+    for (size_t k = 0; k < ctx.size(); ++k) {
+        ses3d_hip_context *c = ctx[k];
+        SectionDev &S = h[k];
+        S.mesh = c->mesh_d; S.evs = c->ev_d; S.itp = c->it_d; S.n_events = c->n_events;
+        S.bpe = elem_blocks(c, sched.epb);
+        S.eb0 = (int)eb;
+        eb += (long long)S.bpe * c->n_events;
+        S.nbpe = (int)node_blocks(c);
+        S.nb0 = (int)nb;
+        nb += (node_blocks(c) * 64 * c->n_events + sched.node_threads - 1) / sched.node_threads;
+        if (eb >= (1LL << 31) || nb >= (1LL << 31)) return fail(SES3D_HIP_EINVAL, "group too large for one launch");
+    }
+    elem_wgs = (int)eb;
+    node_wgs = (int)nb;
+    gens.clear();
+    for (auto *c : ctx) gens.push_back(c->generation);
+    HIP_TRY(hipMemcpyAsync(secs_d, h.data(), sizeof(SectionDev) * h.size(), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
+}
+
+int ses3d_hip_group::launch_element(int mode) {
+    ElemLaunch L{stream, sched.epb, sched.remap_elem, (unsigned)elem_wgs, nullptr, nullptr, nullptr, 0, secs_d,
+                 (int)ctx.size()};
+    return launch_elem(diss, mode, aniso, L);
+}
+
+int ses3d_hip_group::launch_node() {
+    hipLaunchKernelGGL(k_node_group, dim3((unsigned)node_wgs), dim3(sched.node_threads), 0, stream, secs_d,
+                       (int)ctx.size(), sched.remap_node);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+namespace {
 
 int refresh_events(ses3d_hip_context *c);
+
+// run_steps or autotune_steps, with whole-run and per-kernel timing when profiling
+int timed_run(StepRunner *r, int it_begin, int n, int mode, bool record, bool tune) {
+    r->t_elem = r->t_node = r->t_total = 0;
+    r->n_elem = 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (r->profiling) {
+        HIP_TRY(hipEventCreate(&t0));
+        HIP_TRY(hipEventCreate(&t1));
+        HIP_TRY(hipEventRecord(t0, r->stream));
+    }
+    int rc = tune ? autotune_steps(r, it_begin, n, mode, record) : run_steps(r, it_begin, n, mode, record);
+    if (!rc && r->profiling) {
+        rc = hipEventRecord(t1, r->stream) == hipSuccess ? collect_timing(r)
+                                                         : fail(SES3D_HIP_EDEVICE, "hipEventRecord failed");
+        float ms = 0;
+        if (!rc && hipEventElapsedTime(&ms, t0, t1) == hipSuccess) r->t_total = ms;
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    return rc;
+}
+
+int set_schedule(StepRunner *r, const int32_t *v) {
+    Schedule s;
+    s.epb = v[0]; s.node_threads = v[1]; s.remap_elem = v[2]; s.remap_node = v[3];
+    if (!valid_schedule(s))
+        return fail(SES3D_HIP_EINVAL, "schedule: epb in {1,2,4}, node lanes in {128,256,512}, orders in {0,1}");
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    r->sched = s;
+    return r->schedule_changed();
+}
+
+void get_schedule(const StepRunner *r, int32_t *v, float *trial_ms) {
+    v[0] = r->sched.epb; v[1] = r->sched.node_threads; v[2] = r->sched.remap_elem; v[3] = r->sched.remap_node;
+    if (trial_ms)
+        for (int k = 0; k < N_TRIALS; ++k) {
+            trial_ms[2 * k] = r->tuned ? r->tune_ms[k][0] : -1.f;
+            trial_ms[2 * k + 1] = r->tuned ? r->tune_ms[k][1] : -1.f;
+        }
+}
 
 int build_slots(ses3d_hip_context *c, EventHost &E, bool allocate_history) {
     const int nt = c->cfg.nt, sa = c->cfg.samp_ad > 0 ? c->cfg.samp_ad : 1;
@@ -1168,9 +1541,8 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
 void ses3d_hip_destroy(ses3d_hip_context *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
-    for (void *p : c->allocs) (void)hipFree(p);
-    for (auto &e : c->tev) (void)hipEventDestroy(e);
+    c->release_runner();
+    for (auto &a : c->allocs) (void)hipFree(a.first);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1294,11 +1666,9 @@ int ses3d_hip_init(ses3d_hip_context *c) {
         float bh[4] = {c->cfg.tau_p[0], c->cfg.tau_p[1], c->cfg.tau_p[2], 3.0f};
         float yh[4];
         for (int k = 0; k < 4; ++k) yh[k] = 1.0f / bh[k];
-        float *bd = nullptr, *yd = nullptr;
-        unsigned long long *badd = nullptr;
-        TRY(dalloc(c, &bd, 4));
-        TRY(dalloc(c, &yd, 4));
-        TRY(dalloc(c, &badd, 4));
+        if (!c->div_buf) TRY(dalloc(c, &c->div_buf, 8));
+        float *bd = (float *)c->div_buf, *yd = bd + 4;
+        unsigned long long *badd = c->div_buf + 4;
         TRY(upload(c, bd, bh, sizeof(bh)));
         TRY(upload(c, yd, yh, sizeof(yh)));
         HIP_TRY(hipMemsetAsync(badd, 0, sizeof(unsigned long long) * 4, c->stream));
@@ -1319,7 +1689,8 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     // mass matrix
     {
         const long long threads = node_blocks(c) * 64;
-        hipLaunchKernelGGL(k_mass, dim3((unsigned)((threads + NODE_THREADS - 1) / NODE_THREADS)), dim3(NODE_THREADS), 0,
+        hipLaunchKernelGGL(k_mass, dim3((unsigned)((threads + DEFAULT_NODE_THREADS - 1) / DEFAULT_NODE_THREADS)),
+                           dim3(DEFAULT_NODE_THREADS), 0,
                            c->stream, c->mesh_d, c->MM, c->w5);
         HIP_TRY(hipGetLastError());
     }
@@ -1348,8 +1719,8 @@ int ses3d_hip_init(ses3d_hip_context *c) {
         if (af == 0 || af == 1) E.recs = ses3d::locate_receivers(ms, E.rec.data(), (int)(E.rec.size() / 3));
         const int nr = (int)E.recs.size();
         if (nr > 0) {
-            if (!E.seis) TRY(dalloc(c, &E.seis, (size_t)nr * 3 * nt));
-            if (!E.rec_w) TRY(dalloc(c, &E.rec_w, (size_t)nr * 125));
+            TRY(dfit(c, &E.seis, E.seis_cap, (size_t)nr * 3 * nt));
+            TRY(dfit(c, &E.rec_w, E.rec_w_cap, (size_t)nr * 125));
             TRY(zero(c, E.seis, sizeof(float) * (size_t)nr * 3 * nt));
             std::vector<float> w((size_t)nr * 125);
             for (int r = 0; r < nr; ++r) {
@@ -1390,8 +1761,8 @@ int ses3d_hip_init(ses3d_hip_context *c) {
         }
         if (!plist.empty()) {
             const size_t np = plist.size();
-            if (!E.force_w) TRY(dalloc(c, &E.force_w, np * 125));
-            if (!E.force_stf) TRY(dalloc(c, &E.force_stf, np * 3 * (size_t)nt));
+            TRY(dfit(c, &E.force_w, E.force_w_cap, np * 125));
+            TRY(dfit(c, &E.force_stf, E.force_stf_cap, np * 3 * (size_t)nt));
             std::vector<float> wall(np * 125), sall(np * 3 * (size_t)nt);
             for (size_t k = 0; k < np; ++k) {
                 std::copy(plist[k].second.second.begin(), plist[k].second.second.end(), wall.begin() + k * 125);
@@ -1425,51 +1796,52 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     }
     c->nrec_total = (int)recs.size();
     if (!recs.empty()) {
-        if (c->recs_d) { /* reallocated below */ }
-        TRY(dalloc(c, &c->recs_d, recs.size()));
+        TRY(dfit(c, &c->recs_d, c->recs_cap, recs.size()));
         TRY(upload(c, c->recs_d, recs.data(), sizeof(RecDev) * recs.size()));
     }
     c->n_groups = (int)groups.size();
     if (!groups.empty()) {
-        TRY(dalloc(c, &c->groups_d, groups.size()));
-        TRY(dalloc(c, &c->forces_d, forces.size()));
+        TRY(dfit(c, &c->groups_d, c->groups_cap, groups.size()));
+        TRY(dfit(c, &c->forces_d, c->forces_cap, forces.size()));
         TRY(upload(c, c->groups_d, groups.data(), sizeof(ForceGroup) * groups.size()));
         TRY(upload(c, c->forces_d, forces.data(), sizeof(ForceDev) * forces.size()));
     }
     TRY(refresh_events(c));
-    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
-    c->graphs.clear();
+    c->drop_graphs();
     c->it_host = -1;
+    c->generation += 1;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->initialised = true;
     return 0;
 }
 
-int ses3d_hip_run(ses3d_hip_context *c, int32_t it_begin, int32_t n_steps) {
+static int run_checked(ses3d_hip_context *c, int32_t it_begin, int32_t n_steps, bool tune) {
     if (!c) return fail(SES3D_HIP_EINVAL, "null context");
     if (!c->initialised) return fail(SES3D_HIP_ESTATE, "init must precede run");
     if (it_begin < 1 || n_steps < 0 || it_begin + n_steps - 1 > c->cfg.nt)
         return fail(SES3D_HIP_EINVAL, "step range outside 1..nt");
+    if (tune && n_steps < N_TRIALS) return fail(SES3D_HIP_EINVAL, "autotune needs at least 6 steps");
     const int af = c->cfg.adjoint_flag;
     if (af == 1 || af == 2) TRY(ensure_full_history(c));
-    c->t_elem = c->t_node = c->t_total = 0;
-    c->n_elem = 0;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    if (c->profiling) {
-        HIP_TRY(hipEventCreate(&t0));
-        HIP_TRY(hipEventCreate(&t1));
-        HIP_TRY(hipEventRecord(t0, c->stream));
-    }
-    TRY(run_steps(c, it_begin, n_steps, af, af != 2));
-    if (c->profiling) {
-        HIP_TRY(hipEventRecord(t1, c->stream));
-        TRY(collect_timing(c));
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
-        c->t_total = ms;
-        (void)hipEventDestroy(t0);
-        (void)hipEventDestroy(t1);
-    }
+    return timed_run(c, it_begin, n_steps, af, af != 2, tune);
+}
+
+int ses3d_hip_run(ses3d_hip_context *c, int32_t it_begin, int32_t n_steps) {
+    return run_checked(c, it_begin, n_steps, false);
+}
+
+int ses3d_hip_autotune(ses3d_hip_context *c, int32_t it_begin, int32_t n_steps) {
+    return run_checked(c, it_begin, n_steps, true);
+}
+
+int ses3d_hip_set_schedule(ses3d_hip_context *c, const int32_t *sched) {
+    if (!c || !sched) return fail(SES3D_HIP_EINVAL, "null argument");
+    return set_schedule(c, sched);
+}
+
+int ses3d_hip_get_schedule(ses3d_hip_context *c, int32_t *sched, float *trial_ms) {
+    if (!c || !sched) return fail(SES3D_HIP_EINVAL, "null argument");
+    get_schedule(c, sched, trial_ms);
     return 0;
 }
 
@@ -1670,6 +2042,8 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
     // the forward pass keeps the displacement gradient (mode 1) but stores no sampled step
     for (auto &E : c->evh) {
         if ((int)E.ckpt.size() != nseg) {
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            for (float *q : E.ckpt) dfree(c, q);
             E.ckpt.assign(nseg, nullptr);
             for (int s = 0; s < nseg; ++s) TRY(dalloc(c, &E.ckpt[s], ck));
         }
@@ -1717,6 +2091,10 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
     }
     for (auto &E : f->evh)
         if (E.seg_slots < max_slots) {
+            if (E.seg) {
+                HIP_TRY(hipStreamSynchronize(f->stream));
+                dfree(f, E.seg);
+            }
             TRY(dalloc(f, &E.seg, (size_t)max_slots * 9 * NP));
             E.seg_slots = max_slots;
         }
@@ -1776,8 +2154,7 @@ int ses3d_hip_get_info(ses3d_hip_context *c, int64_t *out, int32_t n) {
 int ses3d_hip_set_graph_steps(ses3d_hip_context *c, int32_t steps) {
     if (!c) return fail(SES3D_HIP_EINVAL, "null context");
     if (steps < 0) return fail(SES3D_HIP_EINVAL, "graph step count must be >= 0");
-    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
-    c->graphs.clear();
+    c->drop_graphs();
     c->graph_steps = steps;
     return 0;
 }
@@ -1795,6 +2172,136 @@ int ses3d_hip_get_timing(ses3d_hip_context *c, double *total_ms, double *element
     if (element_ms) *element_ms = c->t_elem;
     if (node_ms) *node_ms = c->t_node;
     if (n_element_launches) *n_element_launches = c->n_elem;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// groups of sections
+// ---------------------------------------------------------------------------
+int ses3d_hip_group_create(ses3d_hip_context *const *ctxs, int32_t n, ses3d_hip_group **out) {
+    if (!ctxs || !out) return fail(SES3D_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (n < 1) return fail(SES3D_HIP_EINVAL, "a group needs at least one context");
+    for (int k = 0; k < n; ++k) {
+        const ses3d_hip_context *c = ctxs[k];
+        if (!c) return fail(SES3D_HIP_EINVAL, "null context in group");
+        if (!c->initialised) return fail(SES3D_HIP_ESTATE, "every member must be initialised before grouping");
+        for (int j = 0; j < k; ++j)
+            if (ctxs[j] == c) return fail(SES3D_HIP_EINVAL, "a context appears twice in the group");
+        if (c->device != ctxs[0]->device) return fail(SES3D_HIP_EINVAL, "members on different devices");
+        if ((c->cfg.is_diss != 0) != (ctxs[0]->cfg.is_diss != 0))
+            return fail(SES3D_HIP_EINVAL, "members must all be elastic or all anelastic");
+        if (c->cfg.adjoint_flag != ctxs[0]->cfg.adjoint_flag)
+            return fail(SES3D_HIP_EINVAL, "members must share adjoint_flag");
+    }
+    auto *g = new ses3d_hip_group();
+    g->ctx.assign(ctxs, ctxs + n);
+    g->diss = ctxs[0]->cfg.is_diss ? 1 : 0;
+    g->af = ctxs[0]->cfg.adjoint_flag;
+    for (auto *c : g->ctx) g->aniso |= c->aniso ? 1 : 0;
+    int rc = 0;
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(SES3D_HIP_EDEVICE, "hipStreamCreate failed");
+    if (!rc && hipMalloc((void **)&g->secs_d, sizeof(SectionDev) * n) != hipSuccess)
+        rc = fail(SES3D_HIP_ENOMEM, "hipMalloc of the group launch table failed");
+    if (!rc) rc = g->build_table();
+    if (rc) {
+        ses3d_hip_group_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return 0;
+}
+
+void ses3d_hip_group_destroy(ses3d_hip_group *g) {
+    if (!g) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    g->release_runner();
+    if (g->secs_d) (void)hipFree(g->secs_d);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+static int group_run_checked(ses3d_hip_group *g, int32_t it_begin, int32_t n_steps, bool tune) {
+    if (!g) return fail(SES3D_HIP_EINVAL, "null group");
+    if (it_begin < 1 || n_steps < 0) return fail(SES3D_HIP_EINVAL, "step range outside 1..nt");
+    if (tune && n_steps < N_TRIALS) return fail(SES3D_HIP_EINVAL, "autotune needs at least 6 steps");
+    int aniso = 0;
+    for (auto *c : g->ctx) {
+        if (!c->initialised) return fail(SES3D_HIP_ESTATE, "every member must be initialised");
+        if (it_begin + n_steps - 1 > c->cfg.nt) return fail(SES3D_HIP_EINVAL, "step range outside 1..nt of a member");
+        aniso |= c->aniso ? 1 : 0;
+    }
+    for (auto *c : g->ctx) {
+        if (g->af == 1 || g->af == 2) TRY(ensure_full_history(c));
+        HIP_TRY(hipStreamSynchronize(c->stream));   // member work issued before this run
+    }
+    bool stale = aniso != g->aniso;
+    for (size_t k = 0; k < g->ctx.size(); ++k) stale = stale || g->gens[k] != g->ctx[k]->generation;
+    g->aniso = aniso;
+    if (stale) TRY(g->schedule_changed());
+    g->it_host = -1;   // members may have stepped on their own since the last group run
+    const int rc = timed_run(g, it_begin, n_steps, g->af, g->af != 2, tune);
+    // later work on a member's own stream starts after this run
+    hipEvent_t done = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(done, g->stream);
+    for (auto *c : g->ctx) {
+        if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, done, 0);
+        c->it_host = rc ? -1 : it_begin + n_steps;
+    }
+    (void)hipEventDestroy(done);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SES3D_HIP_EDEVICE, std::string("group stream ordering: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int ses3d_hip_group_run(ses3d_hip_group *g, int32_t it_begin, int32_t n_steps) {
+    return group_run_checked(g, it_begin, n_steps, false);
+}
+
+int ses3d_hip_group_autotune(ses3d_hip_group *g, int32_t it_begin, int32_t n_steps) {
+    return group_run_checked(g, it_begin, n_steps, true);
+}
+
+int ses3d_hip_group_synchronize(ses3d_hip_group *g) {
+    if (!g) return fail(SES3D_HIP_EINVAL, "null group");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int ses3d_hip_group_set_schedule(ses3d_hip_group *g, const int32_t *sched) {
+    if (!g || !sched) return fail(SES3D_HIP_EINVAL, "null argument");
+    return set_schedule(g, sched);
+}
+
+int ses3d_hip_group_get_schedule(ses3d_hip_group *g, int32_t *sched, float *trial_ms) {
+    if (!g || !sched) return fail(SES3D_HIP_EINVAL, "null argument");
+    get_schedule(g, sched, trial_ms);
+    return 0;
+}
+
+int ses3d_hip_group_set_graph_steps(ses3d_hip_group *g, int32_t steps) {
+    if (!g) return fail(SES3D_HIP_EINVAL, "null group");
+    if (steps < 0) return fail(SES3D_HIP_EINVAL, "graph step count must be >= 0");
+    g->drop_graphs();
+    g->graph_steps = steps;
+    return 0;
+}
+
+int ses3d_hip_group_set_profiling(ses3d_hip_group *g, int32_t on) {
+    if (!g) return fail(SES3D_HIP_EINVAL, "null group");
+    g->profiling = on != 0;
+    return 0;
+}
+
+int ses3d_hip_group_get_timing(ses3d_hip_group *g, double *total_ms, double *element_ms, double *node_ms,
+                               int32_t *n_element_launches) {
+    if (!g) return fail(SES3D_HIP_EINVAL, "null group");
+    if (total_ms) *total_ms = g->t_total;
+    if (element_ms) *element_ms = g->t_elem;
+    if (node_ms) *node_ms = g->t_node;
+    if (n_element_launches) *n_element_launches = g->n_elem;
     return 0;
 }
 

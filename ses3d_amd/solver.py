@@ -82,6 +82,19 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_set_graph_steps": ([P, i32], C.c_int),
         "ses3d_hip_get_info": ([P, C.POINTER(C.c_int64), i32], C.c_int),
         "ses3d_hip_get_timing": ([P, dp, dp, dp, ip], C.c_int),
+        "ses3d_hip_set_schedule": ([P, ip], C.c_int),
+        "ses3d_hip_get_schedule": ([P, ip, fp], C.c_int),
+        "ses3d_hip_autotune": ([P, i32, i32], C.c_int),
+        "ses3d_hip_group_create": ([C.POINTER(P), i32, C.POINTER(P)], C.c_int),
+        "ses3d_hip_group_destroy": ([P], None),
+        "ses3d_hip_group_run": ([P, i32, i32], C.c_int),
+        "ses3d_hip_group_autotune": ([P, i32, i32], C.c_int),
+        "ses3d_hip_group_synchronize": ([P], C.c_int),
+        "ses3d_hip_group_set_schedule": ([P, ip], C.c_int),
+        "ses3d_hip_group_get_schedule": ([P, ip, fp], C.c_int),
+        "ses3d_hip_group_set_graph_steps": ([P, i32], C.c_int),
+        "ses3d_hip_group_set_profiling": ([P, i32], C.c_int),
+        "ses3d_hip_group_get_timing": ([P, dp, dp, dp, ip], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -201,6 +214,22 @@ class Solver:
         if sync:
             self.synchronize()
 
+    def autotune(self, it_begin: int = 1, n_steps: int = 24, sync: bool = True) -> dict:
+        """Advance steps it_begin..it_begin+n_steps-1 (as ``run`` would) while
+        timing the trial launch schedules; keeps the fastest.  Returns
+        ``schedule()``."""
+        _check(self.lib.ses3d_hip_autotune(self.h, it_begin, n_steps))
+        if sync:
+            self.synchronize()
+        return self.schedule()
+
+    def set_schedule(self, epb: int = 2, node_threads: int = 256, remap_elem: int = 1, remap_node: int = 1):
+        v = np.array([epb, node_threads, remap_elem, remap_node], np.int32)
+        _check(self.lib.ses3d_hip_set_schedule(self.h, v.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    def schedule(self) -> dict:
+        return _schedule(self.lib.ses3d_hip_get_schedule, self.h)
+
     def synchronize(self):
         _check(self.lib.ses3d_hip_synchronize(self.h))
 
@@ -290,4 +319,80 @@ class Solver:
         t, e, n = C.c_double(), C.c_double(), C.c_double()
         k = C.c_int32()
         _check(self.lib.ses3d_hip_get_timing(self.h, C.byref(t), C.byref(e), C.byref(n), C.byref(k)))
+        return {"total_ms": t.value, "element_ms": e.value, "node_ms": n.value, "n_element": k.value}
+
+
+def _schedule(fn, h) -> dict:
+    v = np.zeros(4, np.int32)
+    ms = np.zeros(12, np.float32)
+    _check(fn(h, v.ctypes.data_as(C.POINTER(C.c_int32)), _fptr(ms)))
+    return {"epb": int(v[0]), "node_threads": int(v[1]), "remap_elem": int(v[2]), "remap_node": int(v[3]),
+            "trial_ms": ms.reshape(6, 2).tolist() if ms[0] >= 0 else None}
+
+
+class Group:
+    """Sections (initialised Solvers, each with its own mesh and events)
+    advanced together: one element-kernel and one node-kernel launch per step
+    for all of them (ses3d_hip_group_* in include/ses3d_hip.h).  Each member
+    gets exactly the results ``member.run`` would give; outputs are read from
+    the members."""
+
+    def __init__(self, members):
+        self.lib = load_library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        h = C.c_void_p()
+        _check(self.lib.ses3d_hip_group_create(arr, len(self.members), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ses3d_hip_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, it_begin: int = 1, n_steps: Optional[int] = None, sync: bool = True):
+        if n_steps is None:
+            n_steps = min(m.nt for m in self.members) - it_begin + 1
+        _check(self.lib.ses3d_hip_group_run(self.h, it_begin, n_steps))
+        if sync:
+            self.synchronize()
+
+    def autotune(self, it_begin: int = 1, n_steps: int = 24, sync: bool = True) -> dict:
+        _check(self.lib.ses3d_hip_group_autotune(self.h, it_begin, n_steps))
+        if sync:
+            self.synchronize()
+        return self.schedule()
+
+    def synchronize(self):
+        _check(self.lib.ses3d_hip_group_synchronize(self.h))
+
+    def set_schedule(self, epb: int = 2, node_threads: int = 256, remap_elem: int = 1, remap_node: int = 1):
+        v = np.array([epb, node_threads, remap_elem, remap_node], np.int32)
+        _check(self.lib.ses3d_hip_group_set_schedule(self.h, v.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    def schedule(self) -> dict:
+        return _schedule(self.lib.ses3d_hip_group_get_schedule, self.h)
+
+    def set_graph_steps(self, steps: int):
+        _check(self.lib.ses3d_hip_group_set_graph_steps(self.h, int(steps)))
+
+    def set_profiling(self, on: bool):
+        _check(self.lib.ses3d_hip_group_set_profiling(self.h, int(bool(on))))
+
+    def timing(self) -> dict:
+        t, e, n = C.c_double(), C.c_double(), C.c_double()
+        k = C.c_int32()
+        _check(self.lib.ses3d_hip_group_get_timing(self.h, C.byref(t), C.byref(e), C.byref(n), C.byref(k)))
         return {"total_ms": t.value, "element_ms": e.value, "node_ms": n.value, "n_element": k.value}

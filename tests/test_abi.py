@@ -55,3 +55,19 @@ def test_missing_library_fails_loudly(tmp_path):
             solver.load_library(str(tmp_path / "nope.so"))
         finally:
             solver._lib = solver._lib_saved
+
+
+def test_group_and_schedule_entry_points_reject_null():
+    """The group / schedule entry points validate their handles before any
+    device call (no GPU needed)."""
+    from ses3d_amd import solver
+    lib = solver.load_library()
+    h = ctypes.c_void_p()
+    assert lib.ses3d_hip_group_create(None, 1, ctypes.byref(h)) != 0
+    assert b"null" in lib.ses3d_hip_last_error()
+    assert lib.ses3d_hip_group_run(None, 1, 1) != 0
+    assert lib.ses3d_hip_group_autotune(None, 1, 6) != 0
+    sched = (ctypes.c_int32 * 4)(2, 256, 1, 1)
+    assert lib.ses3d_hip_set_schedule(None, sched) != 0
+    assert lib.ses3d_hip_autotune(None, 1, 6) != 0
+    lib.ses3d_hip_group_destroy(None)
