@@ -3,7 +3,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload anelastic_96x96x48]
 
-A step is one iteration of the reference time loop (SOURCE/ses3d_main.f90:184-265)
+A step is one iteration of the reference time loop (SOURCE/ses3d_main.f90:95-178)
 over the whole section: element kernel, point forces, node kernel and the
 seismogram sampling of 200 receivers.  Inputs are resident in HBM before the
 timed region.  updates/s = N_elements * 125 * K / wall.

@@ -99,7 +99,7 @@ int ses3d_hip_set_adjoint_sources(ses3d_hip_context *ctx, int32_t ev, int32_t n,
 int ses3d_hip_init(ses3d_hip_context *ctx);
 
 /* Advance all events by n_steps iterations it_begin .. it_begin+n_steps-1 of
- * the reference time loop (SOURCE/ses3d_main.f90:184-265): the time step
+ * the reference time loop (SOURCE/ses3d_main.f90:95-178): the time step
  * (SOURCE/ses3d_evolution.f90:7-373 with the sources of
  * SOURCE/ses3d_miscellaneous.f90:11-150), seismogram recording
  * (SOURCE/ses3d_output.f90:456-497) in forward modes, forward-field storing
@@ -147,7 +147,7 @@ int ses3d_hip_get_kernels(ses3d_hip_context *ctx, int32_t ev, double *grad_rho, 
 /* Forward-field history (adjoint_flag==1 writes it, adjoint_flag==2 reads
  * it).  A stored step holds 9 float32 fields in reference layout, in the
  * order vx vy vz exx eyy ezz exy exz eyz -- the nine records ses3d_store
- * writes per step (SOURCE/ses3d_main.f90:237-247).  get copies stored step
+ * writes per step (SOURCE/ses3d_main.f90:142-166).  get copies stored step
  * `step` (1-based forward step) out; put uploads one for an adjoint run.
  * saving_vector(it)=it if step it is stored, else 0
  * (SOURCE/ses3d_output.f90:379-392). */

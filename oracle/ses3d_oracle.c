@@ -27,7 +27,7 @@
  *                     make_moment_source (ses3d_miscellaneous.f90:97-150) and
  *                     add_single_force (ses3d_miscellaneous.f90:11-94),
  *                     then record_seismograms (ses3d_output.f90:456-497),
- *                     forward-field store (ses3d_main.f90:231-255,
+ *                     forward-field store (ses3d_main.f90:142-166,
  *                     ses3d_output.f90:5-164), kernel step ses3d_grad
  *                     (ses3d_grad.f90:25-85) and the shear-Q kernel of
  *                     ses3d_grad_Q.f90:111-128,151-153.
@@ -753,7 +753,7 @@ static void o_record(o_sim *o, int it) {
     }
 }
 
-/* forward store, ses3d_main.f90:231-255: v (pre-update), d, symmetric strain */
+/* forward store, ses3d_main.f90:142-166: v (pre-update), d, symmetric strain */
 static void o_store(o_sim *o, int it) {
     int h = o->n_hist++;
     o->hist = (float **)realloc(o->hist, sizeof(float *) * o->n_hist);
@@ -815,7 +815,7 @@ static void o_grad(o_sim *o, o_sim *fwd, int it) {
     }
 }
 
-/* One iteration of the reference time loop (ses3d_main.f90:184-265).
+/* One iteration of the reference time loop (ses3d_main.f90:95-178).
  * fwd: the forward-for-adjoint sim whose stored history the adjoint uses. */
 void o_step(o_sim *o, int it, o_sim *fwd) {
     o_evolution(o, it);

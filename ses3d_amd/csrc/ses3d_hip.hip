@@ -430,7 +430,7 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
         }
 
         if (MODE == 1) {
-            // forward-field store (ses3d_main.f90:237-247)
+            // forward-field store (ses3d_main.f90:142-166)
             const int sl = ldg(E.slot + it);
             if (sl >= 0) {
                 float *h = E.hist + (size_t)sl * 9 * NP;

@@ -13,7 +13,7 @@ Reference readers/writers followed here (file:line):
   relax        SOURCE/ses3d_input.f90:333-357
   stf          SOURCE/ses3d_input.f90:415-430 (4 header lines skipped)
   recfile_<i>  SOURCE/ses3d_input.f90:531-546
-  event_list   SOURCE/ses3d_main.f90:124-132
+  event_list   SOURCE/ses3d_main.f90:35-43
   boxfile      SOURCE/ses3d_input.f90:49-146, writer MODELS/SOURCE/generate_models.f90:210-282
   model files  SOURCE/ses3d_input.f90:472-501, writer MODELS/SOURCE/generate_models.f90:429-452
   seismograms  SOURCE/ses3d_output.f90:398-451, reader TOOLS/seismograms.py:37-98

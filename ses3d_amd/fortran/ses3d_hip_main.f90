@@ -9,7 +9,7 @@
 !   <ofd>/<station>.x/.y/.z, vx_0_<it> ...          (forward)
 !   <ffd>/<ev>/vx_0 ... eyz_0, saving_vector_0      (adjoint_flag=1)
 !   <ofd>/grad_rho_0, grad_cp_0, grad_csh_0, grad_csv_0 (adjoint_flag=2)
-! Structure follows SOURCE/ses3d_main.f90:166-270 with ses3d_input
+! Structure follows SOURCE/ses3d_main.f90:77-182 with ses3d_input
 ! (SOURCE/ses3d_input.f90) restated for one box; the time step, seismograms,
 ! forward-field storage and kernels run in libses3d_hip.so.
 ! The boxfile must describe a single box: the whole section lives on one GPU.
@@ -242,7 +242,7 @@ program ses3d_hip_main
      allocate(vxs(npts), vys(npts), vzs(npts))
      it = 1
      do while (it <= nt)
-        ! the next step whose snapshot is due (SOURCE/ses3d_main.f90:221-225, :269)
+        ! the next step whose snapshot is due (SOURCE/ses3d_main.f90:132-136, :180)
         nsnap = nt + 1
         if (output_displacement == 1) then
            do k = it, nt

@@ -1,6 +1,6 @@
 """Python host side of the MI355X SES3D time-stepper (ctypes over libses3d_hip.so).
 
-Mirrors the reference's run flow (SOURCE/ses3d_main.f90:166-270): inputs as
+Mirrors the reference's run flow (SOURCE/ses3d_main.f90:77-182): inputs as
 ``ses3d_input`` holds them (see :mod:`ses3d_amd.inputs`), ``init`` as
 ``ses3d_init``, ``run`` as the time loop, and outputs in the reference's file
 formats (:mod:`ses3d_amd.formats`).  The library is required: there is no CPU
