@@ -1513,7 +1513,9 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
     if (cfg->adjoint_flag < 0 || cfg->adjoint_flag > 2) return fail(SES3D_HIP_EINVAL, "adjoint_flag must be 0, 1 or 2");
     {
         const long long ne = (long long)(cfg->nx + 1) * (cfg->ny + 1) * (cfg->nz + 1);
-        if (ne * EST + CSPAD >= (1LL << 31)) return fail(SES3D_HIP_EINVAL, "more than 2^31 / 128 elements in one section");
+        // element-local arrays are addressed with 32-bit byte offsets
+        if ((ne * EST + CSPAD) * 4 >= (1LL << 32))
+            return fail(SES3D_HIP_EINVAL, "more than 8.3 million elements in one section");
         const long long ngb = (long long)(cfg->nx + 2) * (cfg->ny + 2) * (cfg->nz + 2) * 64;
         if (ngb * n_events >= (1LL << 31))
             return fail(SES3D_HIP_EINVAL, "events x global nodes exceed 2^31: split the events over contexts");

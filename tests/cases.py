@@ -35,13 +35,13 @@ def make_case(root: str, nel=(6, 7, 4), nt=60, dt=0.2, is_diss=False, adjoint_fl
     return I.load_run(root)
 
 
-def oracle_for(run, ev, adjoint_flag=None, precision=32, nt=None):
+def oracle_for(run, ev, adjoint_flag=None, precision=32, nt=None, pml=2):
     from oracle import pyoracle as PO
     af = run.adjoint_flag if adjoint_flag is None else adjoint_flag
     o = PO.Oracle(nx=run.nx, ny=run.ny, nz=run.nz, xmin=run.xmin, xmax=run.xmax, ymin=run.ymin,
                   ymax=run.ymax, zmin=run.zmin, zmax=run.zmax, zmax_global=run.zmax_global,
                   nt=nt or ev.nt, dt=ev.dt, is_diss=run.is_diss, adjoint_flag=af, samp_ad=run.samp_ad,
-                  pml=2, tau_p=run.tau_p, D_p=run.D_p, precision=precision)
+                  pml=pml, tau_p=run.tau_p, D_p=run.D_p, precision=precision)
     o.set_model(run.model)
     if af in (0, 1):
         o.set_source(ev.source_type, ev.xxs, ev.yys, ev.zzs, ev.moment, ev.stf)
