@@ -297,9 +297,11 @@ def write_run_tree(root: str, case: RunCase) -> dict:
         rec.write(os.path.join(inp, f"recfile_{name}"))
         os.makedirs(os.path.join(root, "MAIN", ev.ofd), exist_ok=True)
     boxes = F.write_boxfile(os.path.join(mod, "boxfile"), case.setup)
-    if len(boxes) != 1:
-        raise NotImplementedError("run trees are written for one rank box (px=py=pz=1)")
-    F.write_model(mod, case.model, 0, bool(case.setup.is_diss))
+    n_r = case.setup.nz_global + case.setup.pz
+    for b in boxes:
+        sl = rank_slices(b, n_r)
+        F.write_model(mod, {k: np.asfortranarray(v[sl]) for k, v in case.model.items()}, b.index - 1,
+                      bool(case.setup.is_diss))
     if case.adjoint:
         for name, (locs, adsrc) in case.adjoint.items():
             d = os.path.join(root, "ADJOINT", name)
@@ -308,6 +310,29 @@ def write_run_tree(root: str, case: RunCase) -> dict:
             for k in range(len(locs)):
                 F.write_ad_src(os.path.join(d, f"ad_src_{k + 1}"), locs[k], adsrc[k])
     return {"boxes": boxes, "names": names}
+
+
+def rank_slices(b: F.Box, n_r: int) -> tuple:
+    """Element ranges of rank box ``b`` in the whole section's element arrays
+    (``n_r`` radial elements in all).
+
+    The reference numbers boxes in an index space of (elements - ranks) per
+    axis, so box i along an axis covers elements nx_min + (i-1) .. nx_max +
+    (i-1) counted from xmin (MODELS/SOURCE/generate_models.f90:200-282,
+    SOURCE/ses3d_input.f90:88-99); radially the count starts at zmin while the
+    element index k grows downwards from zmax (SOURCE/ses3d_init.f90:239)."""
+    (i, j, k) = b.multi
+    z_lo, z_hi = b.nz_min + k - 1, b.nz_max + k - 1          # counted from zmin
+    return (slice(b.nx_min + i - 1, b.nx_max + i), slice(b.ny_min + j - 1, b.ny_max + j),
+            slice(n_r - 1 - z_hi, n_r - z_lo))
+
+
+def decompose(setup: F.Setup, px: int, py: int, pz: int) -> F.Setup:
+    """The same section split into px x py x pz rank boxes (the setup file
+    counts nx_global = elements - px)."""
+    from dataclasses import replace
+    n = (setup.nx_global + setup.px, setup.ny_global + setup.py, setup.nz_global + setup.pz)
+    return replace(setup, nx_global=n[0] - px, ny_global=n[1] - py, nz_global=n[2] - pz, px=px, py=py, pz=pz)
 
 
 def section_setup(n_theta: int, n_phi: int, n_r: int, is_diss: bool = False,

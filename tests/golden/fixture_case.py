@@ -98,3 +98,33 @@ def write_adjoint(fx: dict, root: str, seis: np.ndarray) -> None:
     s = F.Setup.read(os.path.join(root, "INPUT", "setup"))
     s.adjoint_flag = 2
     s.write(os.path.join(root, "INPUT", "setup"))
+
+
+# ---------------------------------------------------------------------------
+# the elastic workload at the reference build's one-rank maximum (23x28x8)
+# ---------------------------------------------------------------------------
+def _elastic_large():
+    from ses3d_amd import configs
+    return configs.Workload("elastic_23x28x8", (23, 28, 8), False, 2, 0.2, 2000, 20, "none")
+
+
+ELASTIC_LARGE = _elastic_large()
+ELASTIC_LARGE_FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "elastic_23x28x8.npz")
+
+
+def input_digest(run) -> str:
+    """sha256 over the float32 inputs of event 1 and the model of a RunInput."""
+    import hashlib
+    h = hashlib.sha256()
+    e = run.events[0]
+    for a in (e.stf, e.moment, e.rec32, np.array([e.xxs, e.yys, e.zzs], np.float32)):
+        h.update(np.ascontiguousarray(a, np.float32).tobytes())
+    for k in sorted(run.model):
+        h.update(np.asarray(run.model[k], np.float32).tobytes(order="F"))
+    return h.hexdigest()
+
+
+def field_digest(a) -> str:
+    """sha256 of a float32 field in the reference's (Fortran) layout."""
+    import hashlib
+    return hashlib.sha256(np.asarray(a, np.float32).tobytes(order="F")).hexdigest()

@@ -144,3 +144,36 @@ def test_reference_binary_live(oracle_lib, tmp_path):
     seis = o.get("seismograms").astype(np.float32)
     for k, n in enumerate(e.receivers.names):
         np.testing.assert_array_equal(F.read_seismogram(ofd, n)["trace"].astype(np.float32), seis[:, k])
+
+
+def test_oracle_matches_reference_elastic_23x28x8(oracle_lib):
+    """The first 200 samples of the reference's 2000-step seismograms at its
+    one-rank maximum mesh (a sample depends only on earlier steps)."""
+    from ses3d_amd import configs
+    from tests.cases import oracle_for
+    from tests.golden.fixture_case import ELASTIC_LARGE, ELASTIC_LARGE_FIXTURE, input_digest
+    fx = np.load(ELASTIC_LARGE_FIXTURE)
+    run = configs.build(ELASTIC_LARGE, seed=0)
+    assert input_digest(run) == str(fx["digest"])
+    o = oracle_for(run, run.events[0], pml=ELASTIC_LARGE.pml)
+    o.run(1, 200)
+    np.testing.assert_array_equal(o.get("seismograms")[:, :, :200].astype(np.float32), fx["seismograms"][:, :, :200])
+
+
+def test_rank_boxes_tile_the_section():
+    """Rank boxes of a decomposed setup cover the section's elements exactly
+    once, with the reference's bottom-up radial box order."""
+    from ses3d_amd import formats as F, models as M
+    s = M.section_setup(24, 24, 12)
+    for ranks in ((2, 1, 2), (3, 2, 1), (1, 1, 3)):
+        d = M.decompose(s, *ranks)
+        cover = np.zeros((24, 24, 12), int)
+        for b in F.make_boxes(d):
+            cover[M.rank_slices(b, 12)] += 1
+            # physical bounds agree with the single-box element grid
+            sl = M.rank_slices(b, 12)
+            dx = np.deg2rad(20.0) / 24
+            assert abs(b.xmin - (np.deg2rad(40.0) + sl[0].start * dx)) < 1e-12
+            dz = 1000e3 / 12
+            assert abs(b.zmax - (6371e3 - sl[2].start * dz)) < 1e-6
+        assert (cover == 1).all(), ranks
