@@ -231,6 +231,60 @@ def batched_leg(n_events: int = 32, steps: int = 30) -> dict:
             "schedule": {k: v for k, v in sched.items() if k != "trial_ms"}, "seismograms_finite": bool(ok)}
 
 
+def inversion_leg(n_events: int = 16, nt: int = 4000, k: int = 100, step_frac: float = 0.005) -> dict:
+    """One inversion iteration on the 64x64x32 anelastic mesh: data from the
+    'true' model (1D + 2% Gaussian blobs of 300 km) for n_events events, the
+    summed L2 misfit gradient at the 1D model through the torch.autograd.Function
+    (checkpointed forward every k steps + adjoint with float64 kernel
+    accumulation, one event per solve: the checkpoints of one event take ~160 GB),
+    a steepest-descent update of vp, vsh, vsv (step_frac of their means along the
+    max-normalised gradient), and the misfit before and after it judged by plain
+    single-event forward solves."""
+    import torch
+    from dataclasses import replace
+    from ses3d_amd import configs
+    from ses3d_amd.autograd import Problem, misfit
+    from ses3d_amd.solver import Solver
+    w = configs.Workload("inversion_64x64x32", (64, 64, 32), True, 2, 0.1, nt, 200, "blobs", n_events=n_events)
+    true = configs.build(w, seed=0)
+    cur = configs.build(replace(w, perturbation="none"), seed=0)
+    t0 = time.perf_counter()
+    f0 = Solver.from_run(true, adjoint_flag=0, pml=w.pml, n_events=n_events)
+    f0.set_model(true.model)
+    for j in range(n_events):
+        f0.load_event(j, true.events[j])
+    f0.init()
+    f0.run()
+    data = [f0.seismograms(j) for j in range(n_events)]
+    f0.close()
+    t_data = time.perf_counter() - t0
+    print(f"# data of {n_events} events in {t_data:.1f} s", file=sys.stderr, flush=True)
+    del true
+    m = cur.model
+    rho0 = 1.0 / m["rhoinv"].astype(np.float64)
+    vs0 = np.sqrt(m["mu"] / rho0)
+    vp0 = np.sqrt((m["lambda"] + 2 * m["mu"]) / rho0)
+    prob = Problem(cur, data, pml=w.pml, ckpt=k, events_per_batch=1, taper_width=40.0,
+                   progress=lambda msg: print(f"# {msg}", file=sys.stderr, flush=True))
+    x = [torch.tensor(a, requires_grad=True) for a in (rho0, vp0, vs0, vs0.copy())]
+    t1 = time.perf_counter()
+    chi0 = misfit(prob, *x)
+    chi0.backward()
+    t_grad = time.perf_counter() - t1
+    g = [t.grad.numpy() for t in x]
+    scale = [0.0, step_frac * vp0.mean(), step_frac * vs0.mean(), step_frac * vs0.mean()]
+    new = [x[i].detach().numpy() - scale[i] * g[i] / max(np.abs(g[i]).max(), 1e-300) for i in range(4)]
+    t2 = time.perf_counter()
+    chi_before = prob.misfit(prob.fields(rho0, vp0, vs0, vs0))
+    chi_after = prob.misfit(prob.fields(*new))
+    t_judge = time.perf_counter() - t2
+    return {"workload": f"inversion iteration, {w.name}, {n_events} events, nt={nt}, dt={w.dt}, checkpoints every {k}",
+            "events": n_events, "gll_points": w.n_points,
+            "s_per_event_gradient": t_grad / n_events, "s_gradient_total": t_grad, "s_data": t_data,
+            "s_judge_two_misfits": t_judge, "misfit_autograd": float(chi0.detach()),
+            "misfit_before": chi_before, "misfit_after": chi_after, "misfit_ratio": chi_after / chi_before}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +296,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--gradient-events", type=int, default=4,
                     help="events solved together in the batched gradient leg (about 57 GB of HBM each)")
+    ap.add_argument("--inversion", action="store_true",
+                    help="only run one inversion iteration on the 64x64x32 mesh (16 events, ~10 min)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a device copy of this many MiB (for HBM counter calibration)")
     args = ap.parse_args()
@@ -258,6 +314,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": 4 * n, "launches": 5}))
+        return
+
+    if args.inversion:
+        print(json.dumps(inversion_leg()))
         return
 
     from ses3d_amd import configs

@@ -63,7 +63,7 @@ class Problem:
 
     def __init__(self, run, data: Sequence[np.ndarray], pml: int = 2, ckpt: int = 100,
                  events_per_batch: Optional[int] = None, q_model: Optional[np.ndarray] = None,
-                 taper_width: float = 0.0):
+                 taper_width: float = 0.0, progress=None):
         self.run = run
         self.data = [np.asarray(d, np.float32) for d in data]
         self.pml = pml
@@ -77,6 +77,7 @@ class Problem:
         self.last_seismograms: List[np.ndarray] = []
         nt = run.events[0].nt
         self.taper = MF.cosine_taper(nt, self.dt, taper_width) if taper_width > 0 else None
+        self.progress = progress          # optional callable(str), told when a batch is done
 
     def fields(self, rho, vp, vsh, vsv, Q=None) -> dict:
         rho, vp, vsh, vsv = (np.asarray(a, np.float64) for a in (rho, vp, vsh, vsv))
@@ -111,6 +112,8 @@ class Problem:
                 self.last_seismograms.append(u)
                 chi += MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
             s.close()
+            if self.progress:
+                self.progress(f"forward solve of events {evs[0]}..{evs[-1]} done")
         return chi
 
     def misfit_and_gradient(self, model):
@@ -142,6 +145,8 @@ class Problem:
                         K[n] += kk[n]
             f.close()
             a.close()
+            if self.progress:
+                self.progress(f"gradient of events {evs[0]}..{evs[-1]} done")
         return chi, K
 
     def gradient_from_kernels(self, K: dict, model: dict) -> dict:

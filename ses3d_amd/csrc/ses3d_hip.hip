@@ -1895,6 +1895,7 @@ int ses3d_hip_get_nr(ses3d_hip_context *c, int32_t ev, int32_t *nr) {
 
 int ses3d_hip_get_receiver_index(ses3d_hip_context *c, int32_t ev, int32_t *idx) {
     TRY(check_ev(c, ev));
+    if (!idx && !c->evh[ev].recs.empty()) return fail(SES3D_HIP_EINVAL, "null idx");
     for (size_t r = 0; r < c->evh[ev].recs.size(); ++r) idx[r] = c->evh[ev].recs[r].index;
     return 0;
 }
@@ -1904,6 +1905,7 @@ int ses3d_hip_get_seismograms(ses3d_hip_context *c, int32_t ev, float *out) {
     EventHost &E = c->evh[ev];
     const int nr = (int)E.recs.size(), nt = c->cfg.nt;
     if (nr == 0) return 0;
+    if (!out) return fail(SES3D_HIP_EINVAL, "null out");
     std::vector<float> tmp((size_t)nr * 3 * nt);
     TRY(download(c, tmp.data(), E.seis, sizeof(float) * tmp.size()));
     for (int r = 0; r < nr; ++r)
@@ -2028,6 +2030,7 @@ int ses3d_hip_get_kernels(ses3d_hip_context *c, int32_t ev, double *g_rho, doubl
 
 int ses3d_hip_get_history(ses3d_hip_context *c, int32_t ev, int32_t step_, float *out9) {
     TRY(check_ev(c, ev));
+    if (!out9) return fail(SES3D_HIP_EINVAL, "null out9");
     EventHost &E = c->evh[ev];
     if (!E.hist_own) return fail(SES3D_HIP_ESTATE, "no stored forward history (run an adjoint_flag=1 context first)");
     if (step_ < 1 || step_ > c->cfg.nt || E.slot_h.empty() || E.slot_h[step_] < 0)
@@ -2043,6 +2046,7 @@ int ses3d_hip_get_history(ses3d_hip_context *c, int32_t ev, int32_t step_, float
 
 int ses3d_hip_put_history(ses3d_hip_context *c, int32_t ev, int32_t step_, const float *in9) {
     TRY(check_ev(c, ev));
+    if (!in9) return fail(SES3D_HIP_EINVAL, "null in9");
     if (!c->initialised) return fail(SES3D_HIP_ESTATE, "init must precede put_history");
     if (c->cfg.adjoint_flag != 2) return fail(SES3D_HIP_ESTATE, "put_history needs an adjoint_flag=2 context");
     TRY(ensure_full_history(c));
