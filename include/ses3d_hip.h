@@ -229,6 +229,19 @@ int ses3d_hip_group_set_profiling(ses3d_hip_group *grp, int32_t on);
 int ses3d_hip_group_get_timing(ses3d_hip_group *grp, double *total_ms, double *element_ms,
                                double *node_ms, int32_t *n_element_launches);
 
+/* Projection of a GLL field onto a block grid, as TOOLS/SOURCE/project_kernel.f90
+ * (:575-715) projects sensitivity kernels for one subvolume: block edges
+ * bx_deg[nbx] (colatitude, degrees), by_deg[nby] (longitude, degrees),
+ * bz_km[nbz] (radius, km), increasing; each block value is the sum over the
+ * elements it overlaps of field * int L_l * int L_m * int L_n (GLL Lagrange
+ * polynomials integrated between the block's limits in each element's local
+ * coordinates) divided by the first block's width product in degrees^2 km.
+ * field: float32 in the reference layout (0:nx,0:ny,0:nz,0:4,0:4,0:4); out:
+ * (nbx-1)(nby-1)(nbz-1) floats, colatitude slowest and radius fastest (the
+ * tool's output order). */
+int ses3d_hip_project(ses3d_hip_context *ctx, const float *field, int32_t nbx, const float *bx_deg,
+                      int32_t nby, const float *by_deg, int32_t nbz, const float *bz_km, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1495,6 +1495,155 @@ int refresh_events(ses3d_hip_context *c) {
 }  // namespace
 
 // ============================================================================
+// kernel projection onto a block grid (TOOLS/SOURCE/project_kernel.f90)
+// ============================================================================
+namespace {
+
+// Integral of the GLL Lagrange polynomial idx (degree 4) over [lo, hi] of the
+// reference interval, float32, as TOOLS/SOURCE/project_kernel.f90:5-163 forms
+// it: coefficients of prod (x - knot_i) by the recurrence, powers by binary
+// exponentiation, terms summed in ascending degree, divided by the product of
+// knot differences.
+float int_lag(int idx, const float *knots, float lo, float hi) {
+    float z0[5];
+    int j = 1;
+    for (int i = 0; i <= 4; ++i)
+        if (i != idx) z0[j++] = -knots[i];
+    float a[5] = {0, 0, 0, 0, 0};
+    a[2] = 1.0f;
+    a[1] = z0[1] + z0[2];
+    a[0] = z0[1] * z0[2];
+    for (int i = 3; i <= 4; ++i) {
+        for (int k = i - 2; k >= 1; --k) a[k] = a[k] * z0[i] + a[k - 1];
+        a[i] = 1.0f;
+        a[i - 1] = 0.0f;
+        a[0] = 1.0f;
+        for (int k = 1; k <= i; ++k) {
+            a[i - 1] = a[i - 1] + z0[k];
+            a[0] = a[0] * z0[k];
+        }
+    }
+    float norm = 1.0f;
+    for (int i = 0; i <= 4; ++i)
+        if (i != idx) norm = norm * (knots[idx] - knots[i]);
+    auto pw = [](float x, int n) {   // binary exponentiation, n >= 1
+        float r = 1.0f;
+        while (true) {
+            if (n & 1) r = r * x;
+            n /= 2;
+            if (n == 0) break;
+            x = x * x;
+        }
+        return r;
+    };
+    float v = 0.0f;
+    for (int i = 0; i <= 4; ++i) v = v + ((pw(hi, i + 1) - pw(lo, i + 1)) * a[i]) / (float)(i + 1);
+    return v / norm;
+}
+
+// index (element) of the GLL coordinate nearest to b, first one in the
+// column-major order of c(0:n, 0:4) (Fortran minloc)
+int nearest_element(const std::vector<float> &c, int n, float b) {
+    int best = 0;
+    float bd = INFINITY;
+    for (int q = 0; q <= 4; ++q)
+        for (int i = 0; i <= n; ++i) {
+            const float d = fabsf(c[i * 5 + q] - b);
+            if (d < bd) { bd = d; best = i; }
+        }
+    return best;
+}
+
+// Per block of one axis: the elements it integrates over and, per element,
+// the 5 integrals of the GLL polynomials between the block's limits in local
+// coordinates (project_kernel.f90:575-690).  Lateral axes count coordinates
+// upwards from cmin; the radial axis (down=true) indexes elements downwards
+// from zmax.
+struct AxisTerms {
+    std::vector<int> start, elem;
+    std::vector<float> w;
+};
+
+AxisTerms axis_terms(const std::vector<float> &c, int n, float cmin, float cmax, float d, const float *knots,
+                     const std::vector<float> &edge, bool down) {
+    AxisTerms T;
+    const int nb = (int)edge.size() - 1;
+    T.start.push_back(0);
+    for (int b = 0; b < nb; ++b) {
+        const float lo = edge[b], hi = edge[b + 1];
+        if (hi >= cmin && lo <= cmax) {
+            int i0, i1;
+            if (!down) {
+                if (lo > cmin) {
+                    i0 = nearest_element(c, n, lo);
+                    if (c[i0 * 5 + 4] == lo) i0 += 1;
+                } else {
+                    i0 = 0;
+                }
+                if (hi < cmax) {
+                    i1 = nearest_element(c, n, hi);
+                    if (c[i1 * 5 + 0] == hi) i1 -= 1;
+                } else {
+                    i1 = n;
+                }
+            } else {
+                i1 = lo > cmin ? nearest_element(c, n, lo) : n;
+                i0 = hi < cmax ? nearest_element(c, n, hi) : 0;
+            }
+            for (int i = i0; i <= i1; ++i) {
+                const float c0 = c[i * 5 + 0], c4 = c[i * 5 + 4];
+                float la, lb;
+                if (!down) {
+                    la = lo > c0 ? ((2.0f * lo - c4) - c0) / d : -1.0f;
+                    lb = hi < c4 ? ((2.0f * hi - c4) - c0) / d : 1.0f;
+                } else {
+                    lb = lo > c4 ? ((c0 + c4) - 2.0f * lo) / d : 1.0f;
+                    la = hi < c0 ? ((c0 + c4) - 2.0f * hi) / d : -1.0f;
+                }
+                T.elem.push_back(i);
+                for (int q = 0; q < 5; ++q) T.w.push_back(int_lag(q, knots, la, lb));
+            }
+        }
+        T.start.push_back((int)T.elem.size());
+    }
+    return T;
+}
+
+struct ProjAxis {
+    const int *start, *elem;
+    const float *w;
+};
+
+// One thread per block: sum over the block's elements (x, y, z order) and
+// their 125 nodes (l, m, n order) of u * intx * inty * intz / vol, float32, in
+// the order of project_kernel.f90:694-711.  u: reference layout.
+__global__ void k_project(const float *__restrict__ u, int NX, int NY, long long NE, ProjAxis X, ProjAxis Y,
+                          ProjAxis Z, int nbx, int nby, int nbz, float vol, float *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbx * nby * nbz) return;
+    const int bz = t % nbz, by = (t / nbz) % nby, bx = t / (nbz * nby);
+    float acc = 0.0f;
+    for (int ix = X.start[bx]; ix < X.start[bx + 1]; ++ix)
+        for (int iy = Y.start[by]; iy < Y.start[by + 1]; ++iy)
+            for (int iz = Z.start[bz]; iz < Z.start[bz + 1]; ++iz) {
+                const long long e = X.elem[ix] + (long long)NX * (Y.elem[iy] + (long long)NY * Z.elem[iz]);
+                for (int l = 0; l < 5; ++l) {
+                    const float wx = X.w[ix * 5 + l];
+                    for (int m = 0; m < 5; ++m) {
+                        const float wy = Y.w[iy * 5 + m];
+                        for (int n = 0; n < 5; ++n) {
+                            const float wz = Z.w[iz * 5 + n];
+                            acc = acc + (((u[e + NE * (l + 5 * (m + 5 * n))] * wx) * wy) * wz) / vol;
+                        }
+                    }
+                }
+            }
+    out[t] = acc;
+}
+
+}  // namespace
+
+// ============================================================================
 // C-ABI
 // ============================================================================
 extern "C" {
@@ -2341,6 +2490,60 @@ int ses3d_hip_group_get_timing(ses3d_hip_group *g, double *total_ms, double *ele
     if (element_ms) *element_ms = g->t_elem;
     if (node_ms) *node_ms = g->t_node;
     if (n_element_launches) *n_element_launches = g->n_elem;
+    return 0;
+}
+
+int ses3d_hip_project(ses3d_hip_context *c, const float *field, int32_t nbx, const float *bx_deg, int32_t nby,
+                      const float *by_deg, int32_t nbz, const float *bz_km, float *out) {
+    if (!c || !field || !bx_deg || !by_deg || !bz_km || !out) return fail(SES3D_HIP_EINVAL, "null argument");
+    if (nbx < 2 || nby < 2 || nbz < 2) return fail(SES3D_HIP_EINVAL, "a block grid needs at least 2 edges per axis");
+    const ses3d::MeshSetup &ms = c->ms;
+    // edges as project_kernel.f90:381-388 converts them (pi of SOURCE/ses3d_modules.f90:25)
+    const float pi = 3.1415926535898f;
+    std::vector<float> ex(nbx), ey(nby), ez(nbz);
+    for (int i = 0; i < nbx; ++i) ex[i] = (bx_deg[i] * pi) / 180.0f;
+    for (int i = 0; i < nby; ++i) ey[i] = (by_deg[i] * pi) / 180.0f;
+    for (int i = 0; i < nbz; ++i) ez[i] = bz_km[i] * 1000.0f;
+    const float vol = ((bx_deg[1] - bx_deg[0]) * (by_deg[1] - by_deg[0])) * (bz_km[1] - bz_km[0]);
+    const AxisTerms TX = axis_terms(ms.x, ms.nx, ms.xmin, ms.xmax, ms.dx, ms.knots, ex, false);
+    const AxisTerms TY = axis_terms(ms.y, ms.ny, ms.ymin, ms.ymax, ms.dy, ms.knots, ey, false);
+    const AxisTerms TZ = axis_terms(ms.z, ms.nz, ms.zmin, ms.zmax, ms.dz, ms.knots, ez, true);
+    const long long nb = (long long)(nbx - 1) * (nby - 1) * (nbz - 1);
+    if (nb >= (1LL << 31)) return fail(SES3D_HIP_EINVAL, "block grid too large");
+    // one device buffer for the field, the tables and the output (freed before return)
+    const AxisTerms *T[3] = {&TX, &TY, &TZ};
+    size_t bytes = sizeof(float) * ((size_t)c->NPTS + (size_t)nb);
+    for (auto *t : T) bytes += sizeof(int) * (t->start.size() + t->elem.size() + 1) + sizeof(float) * (t->w.size() + 1);
+    char *buf = nullptr;
+    HIP_TRY(hipMalloc((void **)&buf, bytes));
+    int rc = 0;
+    // field, output, then per axis: block starts, element indices, integrals
+    float *u_d = (float *)buf;
+    float *out_d = u_d + c->NPTS;
+    int *ints = (int *)(out_d + nb);
+    ProjAxis PA[3];
+    for (int a = 0; a < 3; ++a) {
+        const AxisTerms &t = *T[a];
+        int *st = ints;
+        int *el = st + t.start.size();
+        float *w = (float *)(el + t.elem.size() + 1);
+        ints = (int *)(w + t.w.size() + 1);
+        PA[a] = {st, el, w};
+        if (!rc && hipMemcpyAsync(st, t.start.data(), sizeof(int) * t.start.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
+        if (!rc && !t.elem.empty() && hipMemcpyAsync(el, t.elem.data(), sizeof(int) * t.elem.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
+        if (!rc && !t.w.empty() && hipMemcpyAsync(w, t.w.data(), sizeof(float) * t.w.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
+    }
+    if (!rc && hipMemcpyAsync(u_d, field, sizeof(float) * c->NPTS, hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
+    if (!rc) {
+        hipLaunchKernelGGL(k_project, dim3((unsigned)((nb + 127) / 128)), dim3(128), 0, c->stream, u_d, c->NX, c->NY,
+                           c->NE, PA[0], PA[1], PA[2], nbx - 1, nby - 1, nbz - 1, vol, out_d);
+        if (hipGetLastError() != hipSuccess) rc = 1;
+    }
+    if (!rc && hipMemcpyAsync(out, out_d, sizeof(float) * nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess) rc = 1;
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = 1;
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    if (rc) return fail(SES3D_HIP_EDEVICE, "kernel projection failed on the device");
     return 0;
 }
 

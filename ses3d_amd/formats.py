@@ -615,3 +615,53 @@ def write_saving_vector(path: str, sv: np.ndarray) -> None:
 def read_saving_vector(path: str, nt: int) -> np.ndarray:
     L = _Lines(path)
     return np.array([int(L.value()) for _ in range(nt)], np.int32)
+
+
+# ---------------------------------------------------------------------------
+# block grids and projected kernels (TOOLS/SOURCE/project_kernel.f90)
+# ---------------------------------------------------------------------------
+
+def write_block_file(path: str, edges: Sequence[np.ndarray]) -> None:
+    """MODELS/MODELS_3D/block_{x,y,z}: the number of subvolumes, then per
+    subvolume the number of block edges and one edge per line (degrees for
+    x = colatitude and y = longitude, km of radius for z), as
+    TOOLS/SOURCE/project_kernel.f90:306-308,350-372 reads them."""
+    out = [f"{len(edges)}"]
+    for e in edges:
+        out.append(f"{len(e)}")
+        out += [repr(float(v)) for v in e]
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+
+
+def read_block_file(path: str) -> List[np.ndarray]:
+    toks = [_tok(l) for l in open(path) if l.strip()]
+    n, k, out = int(toks[0]), 1, []
+    for _ in range(n):
+        m = int(toks[k]); k += 1
+        out.append(np.array([float(t) for t in toks[k:k + m]]))
+        k += m
+    return out
+
+
+def read_projected(path: str) -> List[np.ndarray]:
+    """gradient_<param> written by project_kernel.f90:735-750: the number of
+    subvolumes, then per subvolume the block count and one value per block
+    (x slowest, z fastest)."""
+    toks = [_tok(l) for l in open(path) if l.strip()]
+    n, k, out = int(toks[0]), 1, []
+    for _ in range(n):
+        m = int(toks[k]); k += 1
+        out.append(np.array([float(t) for t in toks[k:k + m]], np.float32))
+        k += m
+    return out
+
+
+def write_projected(path: str, values: Sequence[np.ndarray]) -> None:
+    out = [f"{len(values)}"]
+    for v in values:
+        v = np.asarray(v, np.float32).ravel()
+        out.append(f"{v.size}")
+        out += [_fortran_real(x) for x in v]
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")

@@ -95,6 +95,7 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_group_set_graph_steps": ([P, i32], C.c_int),
         "ses3d_hip_group_set_profiling": ([P, i32], C.c_int),
         "ses3d_hip_group_get_timing": ([P, dp, dp, dp, ip], C.c_int),
+        "ses3d_hip_project": ([P, fp, i32, fp, i32, fp, i32, fp, fp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -297,6 +298,26 @@ class Solver:
     def adjoint_checkpointed(self, fwd: "Solver"):
         _check(self.lib.ses3d_hip_adjoint_checkpointed(self.h, fwd.h))
         self.synchronize()
+
+    def project(self, field, bx_deg, by_deg, bz_km) -> np.ndarray:
+        """Block-grid projection of a reference-layout field as
+        TOOLS/SOURCE/project_kernel.f90 projects a kernel (one subvolume);
+        returns (nbx-1, nby-1, nbz-1) float32."""
+        f = _ref_f32(field)
+        e = [np.ascontiguousarray(a, np.float32) for a in (bx_deg, by_deg, bz_km)]
+        out = np.zeros((len(e[0]) - 1) * (len(e[1]) - 1) * (len(e[2]) - 1), np.float32)
+        _check(self.lib.ses3d_hip_project(self.h, _fptr(f), len(e[0]), _fptr(e[0]), len(e[1]), _fptr(e[1]),
+                                          len(e[2]), _fptr(e[2]), _fptr(out)))
+        return out.reshape(len(e[0]) - 1, len(e[1]) - 1, len(e[2]) - 1)
+
+    def project_kernels(self, bx_deg, by_deg, bz_km, ev: int = 0) -> dict:
+        """This context's kernels, rounded to float32 as the reference writes
+        them, projected onto the block grid: {'gradient_csh', 'gradient_csv',
+        'gradient_cp', 'gradient_rho'[, 'gradient_Q_mu']}."""
+        K = self.kernels(ev)
+        names = {"grad_csh": "gradient_csh", "grad_csv": "gradient_csv", "grad_cp": "gradient_cp",
+                 "grad_rho": "gradient_rho", "grad_Q_mu": "gradient_Q_mu"}
+        return {names[k]: self.project(v.astype(np.float32), bx_deg, by_deg, bz_km) for k, v in K.items()}
 
     def memory_bytes(self) -> int:
         b = C.c_int64()

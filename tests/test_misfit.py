@@ -30,3 +30,17 @@ def test_misfit_matches_definition():
     assert np.isclose(MF.l2_misfit(u, d, 0.1), 0.5 * np.sum((u - d) ** 2) * 0.1)
     w = MF.cosine_taper(30, 0.1, 1.0)
     assert np.isclose(MF.l2_misfit(u, d, 0.1, w), 0.5 * np.sum(((u - d) * w) ** 2) * 0.1)
+
+
+def test_block_and_projection_files_round_trip(tmp_path):
+    """MODELS_3D/block_* and projected gradient files (the layouts
+    TOOLS/SOURCE/project_kernel.f90 reads and writes) round-trip."""
+    from ses3d_amd import formats as F
+    edges = [np.linspace(40.0, 60.0, 11), np.array([45.0, 47.5, 50.0])]
+    F.write_block_file(str(tmp_path / "block_x"), edges)
+    back = F.read_block_file(str(tmp_path / "block_x"))
+    assert len(back) == 2 and all(np.array_equal(a, b) for a, b in zip(back, edges))
+    vals = [np.array([1.5, -2.25e-30, 3.0e12], np.float32)]
+    F.write_projected(str(tmp_path / "gradient_csh"), vals)
+    got = F.read_projected(str(tmp_path / "gradient_csh"))
+    assert np.array_equal(got[0], vals[0])
