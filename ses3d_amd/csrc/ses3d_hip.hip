@@ -106,9 +106,9 @@ struct MeshDev {
 
 struct EventDev {
     float *v;          // 3*NGB
-    float *d;          // 9*NP  dxux dyuy dzuz dxuy dyux dxuz dzux dyuz dzuy
-    float *s;          // 9*NP  sxx syy szz sxy syx sxz szx syz szy (split stresses)
-    float *M;          // 18*NP [c][k] c: xx yy zz xy xz yz, k: mechanism
+    float *d;          // 9*NP  dxux dyuy dzuz dxuy dyux dxuz dzux dyuz dzuy (paired, c9_offset)
+    float *s;          // 9*NP  sxx syy szz sxy syx sxz szx syz szy (split stresses; paired, c9_offset)
+    float *M;          // 18*NP index c*3+k, c: xx yy zz xy xz yz, k: mechanism (paired)
     float *f;          // 3*NP  element force
     int src_elem;      // moment-tensor source element or -1
     const float *src_md;   // 9*125 MOM_c*delta
@@ -156,6 +156,58 @@ __device__ __forceinline__ float ldo(const float *base, unsigned boff) {
     return *(cgfloat *)((cgchar *)base + boff);
 }
 __device__ __forceinline__ void sto(float *base, unsigned boff, float v) { *(gfloat *)((gchar *)base + boff) = v; }
+
+// Paired components: two components of one point side by side, so a lane
+// moves them with one 8-byte access (a wavefront then moves 512 contiguous
+// bytes per instruction with half the memory instructions).  A pair plane
+// holds 2*NP floats, point p's pair at floats 2p, 2p+1.
+//   d, s (9 components): planes 0..3 = (EXX,EYY) (EZZ,EXY) (EXZ,EYZ) (EYX,EZX),
+//                        EZY alone at float 8*NP (stride 1);
+//   M (18 components, index c*3+k): plane q = (2q, 2q+1).
+// Outside the absorbing layers only planes 0..2 of s are touched.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const f2v cgf2v;
+typedef __attribute__((address_space(1))) f2v gf2v;
+__device__ __forceinline__ f2v ldo2(const float *base, unsigned boff) {
+    return *(cgf2v *)((cgchar *)base + boff);
+}
+__device__ __forceinline__ void sto2(float *base, unsigned boff, float a, float b) {
+    f2v v;
+    v.x = a;
+    v.y = b;
+    *(gf2v *)((gchar *)base + boff) = v;
+}
+// float offset of component c's first point in a 9-component array, and
+// the stride between points (2 in a pair plane, 1 for EZY)
+__host__ __device__ constexpr long long c9_offset(int c, long long NP) {
+    return c == EXX ? 0 : c == EYY ? 1 : c == EZZ ? 2 * NP : c == EXY ? 2 * NP + 1 : c == EXZ ? 4 * NP
+         : c == EYZ ? 4 * NP + 1 : c == EYX ? 6 * NP : c == EZX ? 6 * NP + 1 : 8 * NP;
+}
+__host__ __device__ constexpr int c9_stride(int c) { return c == EZY ? 1 : 2; }
+// inner: the six components every element touches (planes 0..2)
+__device__ __forceinline__ void ld9_inner(const float *b, size_t NP, unsigned pb2, float *x) {
+    f2v t = ldo2(b, pb2);
+    x[EXX] = t.x; x[EYY] = t.y;
+    t = ldo2(b + 2 * NP, pb2);
+    x[EZZ] = t.x; x[EXY] = t.y;
+    t = ldo2(b + 4 * NP, pb2);
+    x[EXZ] = t.x; x[EYZ] = t.y;
+}
+// outer: the second members of the split pairs (plane 3 and EZY)
+__device__ __forceinline__ void ld9_outer(const float *b, size_t NP, unsigned pb2, unsigned pb, float *x) {
+    const f2v t = ldo2(b + 6 * NP, pb2);
+    x[EYX] = t.x; x[EZX] = t.y;
+    x[EZY] = ldo(b + 8 * NP, pb);
+}
+__device__ __forceinline__ void st9_inner(float *b, size_t NP, unsigned pb2, const float *x) {
+    sto2(b, pb2, x[EXX], x[EYY]);
+    sto2(b + 2 * NP, pb2, x[EZZ], x[EXY]);
+    sto2(b + 4 * NP, pb2, x[EXZ], x[EYZ]);
+}
+__device__ __forceinline__ void st9_outer(float *b, size_t NP, unsigned pb2, unsigned pb, const float *x) {
+    sto2(b + 6 * NP, pb2, x[EYX], x[EZX]);
+    sto(b + 8 * NP, pb, x[EZY]);
+}
 
 __device__ __forceinline__ void stnt(float *p, float v) {
 #if SES3D_NT_STORE
@@ -236,6 +288,7 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
     const int ei = e % NX, ej = (e / NX) % NY, ek = e / (NX * NY);
     const int p = e * EST + nd;
     const unsigned pb = 4u * (unsigned)p;   // byte offset in every element-local array
+    const unsigned pb2 = 2u * pb;           // byte offset in the pair planes
     // element in an absorbing layer? (uniform over the element's wavefronts)
     const bool pml = act && (ldg(Ms.PMLX + ei) | ldg(Ms.PMLY + ej) | ldg(Ms.PMLZ + ek));
     const bool keep_d = (MODE != 0) || pml;
@@ -251,23 +304,18 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
         vy = ldnt(E.v + Ms.NGB + g);
         vz = ldnt(E.v + 2 * Ms.NGB + g);
         if (keep_d) {
-#pragma unroll
-            for (int c = 0; c < 9; ++c) d9[c] = ldo(E.d + c * NP, pb);
+            ld9_inner(E.d, NP, pb2, d9);
+            ld9_outer(E.d, NP, pb2, pb, d9);
         }
-        s9[EXX] = ldo(E.s + EXX * NP, pb);
-        s9[EYY] = ldo(E.s + EYY * NP, pb);
-        s9[EZZ] = ldo(E.s + EZZ * NP, pb);
-        s9[EXY] = ldo(E.s + EXY * NP, pb);
-        s9[EXZ] = ldo(E.s + EXZ * NP, pb);
-        s9[EYZ] = ldo(E.s + EYZ * NP, pb);
-        if (pml) {
-            s9[EYX] = ldo(E.s + EYX * NP, pb);
-            s9[EZX] = ldo(E.s + EZX * NP, pb);
-            s9[EZY] = ldo(E.s + EZY * NP, pb);
-        }
+        ld9_inner(E.s, NP, pb2, s9);
+        if (pml) ld9_outer(E.s, NP, pb2, pb, s9);
         if (DISS) {
 #pragma unroll
-            for (int c = 0; c < 18; ++c) Mv[c] = ldo(E.M + c * NP, pb);
+            for (int q = 0; q < 9; ++q) {
+                const f2v t = ldo2(E.M + 2 * q * NP, pb2);
+                Mv[2 * q] = t.x;
+                Mv[2 * q + 1] = t.y;
+            }
             QQ = ldo(Ms.Q, pb);
         }
         lam = ldo(Ms.lam, pb);
@@ -326,9 +374,11 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             dxuy = d9[EXY] + dt * exy;
             dyux = d9[EYX] + dt * eyx;
             float *d = E.d;
-            sto(d + EXX * NP, pb, dxux); sto(d + EYY * NP, pb, dyuy); sto(d + EZZ * NP, pb, dzuz);
-            sto(d + EXY * NP, pb, dxuy); sto(d + EYX * NP, pb, dyux); sto(d + EXZ * NP, pb, dxuz);
-            sto(d + EZX * NP, pb, dzux); sto(d + EYZ * NP, pb, dyuz); sto(d + EZY * NP, pb, dzuy);
+            sto2(d, pb2, dxux, dyuy);
+            sto2(d + 2 * NP, pb2, dzuz, dxuy);
+            sto2(d + 4 * NP, pb2, dxuz, dyuz);
+            sto2(d + 6 * NP, pb2, dyux, dzux);
+            sto(d + 8 * NP, pb, dzuy);
         }
         // absorbing-layer terms (:123-134); exact no-ops outside the layers
         float px = 0.f, py = 0.f, pz = 0.f, taper = 1.f;
@@ -393,7 +443,7 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
                     const float q = Ms.fast_tau ? div_recip(dM, Ms.tau_p[k], Ms.rtau_p[k]) : dM / Ms.tau_p[k];
                     const float Mn = (Mo - q) - Ms.Lk[k] * ev6[c];
                     Mv[c * 3 + k] = Mn;
-                    sto(E.M + (c * 3 + k) * NP, pb, Mn);
+                    if ((c * 3 + k) & 1) sto2(E.M + (c * 3 + k - 1) * NP, pb2, Mv[c * 3 + k - 1], Mn);
                 }
         } else {
             const float L = lam * ((exx + eyy) + ezz);
@@ -454,10 +504,11 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
         // the untapered stresses
         if (pml) {
 #pragma unroll
-            for (int c = 0; c < 9; ++c) sto(s + c * NP, pb, s9[c] * taper);
+            for (int c = 0; c < 9; ++c) s9[c] = s9[c] * taper;
+            st9_inner(s, NP, pb2, s9);
+            st9_outer(s, NP, pb2, pb, s9);
         } else {
-            sto(s + EXX * NP, pb, s9[EXX]); sto(s + EYY * NP, pb, s9[EYY]); sto(s + EZZ * NP, pb, s9[EZZ]);
-            sto(s + EXY * NP, pb, s9[EXY]); sto(s + EXZ * NP, pb, s9[EXZ]); sto(s + EYZ * NP, pb, s9[EYZ]);
+            st9_inner(s, NP, pb2, s9);
         }
 
         if (MODE == 1) {
@@ -1663,8 +1714,9 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
     {
         const long long ne = (long long)(cfg->nx + 1) * (cfg->ny + 1) * (cfg->nz + 1);
         // element-local arrays are addressed with 32-bit byte offsets
-        if ((ne * EST + CSPAD) * 4 >= (1LL << 32))
-            return fail(SES3D_HIP_EINVAL, "more than 8.3 million elements in one section");
+        // (pair planes hold 2 floats per point)
+        if ((ne * EST + CSPAD) * 8 >= (1LL << 32))
+            return fail(SES3D_HIP_EINVAL, "more than 4.1 million elements in one section");
         const long long ngb = (long long)(cfg->nx + 2) * (cfg->ny + 2) * (cfg->nz + 2) * 64;
         if (ngb * n_events >= (1LL << 31))
             return fail(SES3D_HIP_EINVAL, "events x global nodes exceed 2^31: split the events over contexts");
@@ -2073,6 +2125,17 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
     static const char *sn[9] = {"sxx", "syy", "szz", "sxy", "syx", "sxz", "szx", "syz", "szy"};
     static const char *mn[6] = {"Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz"};
     std::vector<float> tmp;
+    // one component of a paired array: download its plane and keep every
+    // stride-th float from the component's offset
+    auto component = [&](std::vector<float> &dst, const float *base, long long off, int stride) -> int {
+        dst.resize(NP);
+        if (stride == 1) return download(c, dst.data(), base + off, sizeof(float) * NP);
+        const long long plane = off & ~1LL;
+        std::vector<float> pl(2 * NP);
+        TRY(download(c, pl.data(), base + plane, sizeof(float) * 2 * NP));
+        for (long long i = 0; i < NP; ++i) dst[i] = pl[2 * i + (off - plane)];
+        return 0;
+    };
     // outside the absorbing layers the second member of each split-stress pair
     // is the first (only the first is stored there)
     static const int pair[9] = {EXX, EYY, EZZ, EXY, EXY, EXZ, EXZ, EYZ, EYZ};
@@ -2081,17 +2144,15 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
             if (c->cfg.adjoint_flag == 0)
                 return fail(SES3D_HIP_EINVAL, "with adjoint_flag=0 the displacement gradient is integrated only "
                                               "inside the absorbing layers (the only place it acts); use adjoint_flag 1 or 2");
-            tmp.resize(NP);
-            TRY(download(c, tmp.data(), E.d + k * NP, sizeof(float) * NP));
+            TRY(component(tmp, E.d, c9_offset(k, NP), c9_stride(k)));
             em_to_ref(c, tmp.data(), out);
             return 0;
         }
         if (nm == sn[k]) {
-            tmp.resize(NP);
-            TRY(download(c, tmp.data(), E.s + k * NP, sizeof(float) * NP));
+            TRY(component(tmp, E.s, c9_offset(k, NP), c9_stride(k)));
             if (pair[k] != k) {
-                std::vector<float> first(NP);
-                TRY(download(c, first.data(), E.s + pair[k] * NP, sizeof(float) * NP));
+                std::vector<float> first;
+                TRY(component(first, E.s, c9_offset(pair[k], NP), c9_stride(pair[k])));
                 for (long long e = 0; e < c->NE; ++e) {
                     const int i = (int)(e % c->NX), j = (int)((e / c->NX) % c->NY), kk = (int)(e / ((long long)c->NX * c->NY));
                     if (!(c->pmlx[i] | c->pmly[j] | c->pmlz[kk]))
@@ -2105,9 +2166,9 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
     for (int k = 0; k < 6; ++k)
         if (nm == mn[k]) {
             if (!E.M) return fail(SES3D_HIP_EINVAL, "memory variables exist only when is_diss=1");
-            tmp.resize(NP);
             for (int mech = 0; mech < 3; ++mech) {
-                TRY(download(c, tmp.data(), E.M + (k * 3 + mech) * NP, sizeof(float) * NP));
+                const int i = k * 3 + mech;   // pair plane i / 2, member i % 2
+                TRY(component(tmp, E.M, (long long)(i & ~1) * NP + (i & 1), 2));
                 em_to_ref(c, tmp.data(), out + mech * c->NPTS);
             }
             return 0;
