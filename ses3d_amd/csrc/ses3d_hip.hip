@@ -92,7 +92,8 @@ struct MeshDev {
     float dt, Jac, sum_D_p;
     float tau_p[3], Lk[3];
     float tab[7 * 25];         // D1 D2 D3 WDL DUM1 DUM2 DUM3
-    const float *lam, *mu, *A, *B, *C, *Q, *rhoinv, *MM;
+    const float *lm;   // 2*NP: (lambda, mu) pairs
+    const float *A, *B, *C, *Q, *rhoinv, *MM;
     const float *Z, *S, *COT;
     const unsigned short *IX, *IY, *IZ;
     const float *LUT;
@@ -109,7 +110,7 @@ struct EventDev {
     float *d;          // 9*NP  dxux dyuy dzuz dxuy dyux dxuz dzux dyuz dzuy (paired, c9_offset)
     float *s;          // 9*NP  sxx syy szz sxy syx sxz szx syz szy (split stresses; paired, c9_offset)
     float *M;          // 18*NP index c*3+k, c: xx yy zz xy xz yz, k: mechanism (paired)
-    float *f;          // 3*NP  element force
+    float *f;          // 3*NP  element force: (fx, fy) pair plane, then fz at 2*NP
     int src_elem;      // moment-tensor source element or -1
     const float *src_md;   // 9*125 MOM_c*delta
     const float *so;       // nt source time function
@@ -318,8 +319,11 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             }
             QQ = ldo(Ms.Q, pb);
         }
-        lam = ldo(Ms.lam, pb);
-        mu = ldo(Ms.mu, pb);
+        {
+            const f2v t = ldo2(Ms.lm, pb2);
+            lam = t.x;
+            mu = t.y;
+        }
         if (ANISO) {
             A = ldo(Ms.A, pb);
             B = ldo(Ms.B, pb);
@@ -586,8 +590,7 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             ay = ((ay + ss[sub][czy][pz_] * wz) + ss[sub][EYY][py_] * wy) + ss[sub][EXY][px_] * wx;
             az = ((az + ss[sub][EZZ][pz_] * wz) + ss[sub][EYZ][py_] * wy) + ss[sub][EXZ][px_] * wx;
         }
-        sto(E.f, pb, ax);
-        sto(E.f + NP, pb, ay);
+        sto2(E.f, pb2, ax, ay);
         sto(E.f + 2 * NP, pb, az);
     }
 }
@@ -668,7 +671,7 @@ __global__ void k_forces(const MeshDev *__restrict__ mesh, const EventDev *__res
     const long long NP = mesh->NP;
     float *f = evs[G.ev].f;
     const long long p = G.elem * EST + nd;
-    float fx = f[p], fy = f[NP + p], fz = f[2 * NP + p];
+    float fx = f[2 * p], fy = f[2 * p + 1], fz = f[2 * NP + p];
     for (int s = G.first; s < G.first + G.count; ++s) {
         const ForceDev F = forces[s];
         const float w = F.w[nd];
@@ -676,7 +679,7 @@ __global__ void k_forces(const MeshDev *__restrict__ mesh, const EventDev *__res
         if (F.comp_mask & 2) fy = fy - w * F.stf[1 * nt + it - 1];
         if (F.comp_mask & 4) fz = fz - w * F.stf[2 * nt + it - 1];
     }
-    f[p] = fx; f[NP + p] = fy; f[2 * NP + p] = fz;
+    f[2 * p] = fx; f[2 * p + 1] = fy; f[2 * NP + p] = fz;
 }
 
 // ---------------------------------------------------------------------------
@@ -685,22 +688,14 @@ __global__ void k_forces(const MeshDev *__restrict__ mesh, const EventDev *__res
 // (:326-328), taper (:366).  One thread per global node; a wavefront owns one
 // 4x4x4 node block.
 // ---------------------------------------------------------------------------
-// `gt`: the lane's index among the section's n_events * nodeblocks_per_event * 64
-// node lanes (32-bit: n_events * padded nodes stays below 2^31, checked on the host).
-template <typename MT, typename ET>
-__device__ __forceinline__ void node_update(MT &Ms, ET *__restrict__ evs, const int nodeblocks_per_event,
-                                            const int n_events, const int gt) {
-    const int nbt = gt >> 6;
-    const int ev = nbt / nodeblocks_per_event;
-    if (ev >= n_events) return;
-    const int nb = nbt - ev * nodeblocks_per_event;
-    const int loc = gt & 63;
+// One node block `nb` (x fastest, then y, then z) of event E; lane `loc`.
+template <typename MT, typename EV>
+__device__ __forceinline__ void node_block(MT &Ms, EV &E, const int nb, const int loc) {
     const int NX = Ms.NX, NY = Ms.NY, NZ = Ms.NZ, NBX = Ms.NBX, NBY = Ms.NBY;
     const int byz = nb / NBX;
     const int bx = nb - byz * NBX, bz = byz / NBY, by = byz - bz * NBY;
     const int a = loc & 3, b = (loc >> 2) & 3, c = loc >> 4;
     if ((bx == NX && a) || (by == NY && b) || (bz == NZ && c)) return;
-    auto &E = evs[ev];
     const size_t NP = (size_t)Ms.NP;
     // sharers per axis in ascending element index: (e0, l0) and, on an
     // element face, (e1, l1); held in scalars so nothing goes to scratch
@@ -733,8 +728,9 @@ __device__ __forceinline__ void node_update(MT &Ms, ET *__restrict__ evs, const 
                     const int ey = jj ? by : ye0, ly = jj ? 0 : yl0;
                     const int ez = kk ? bz : ze0, lz = kk ? 0 : zl0;
                     const int q = (ex + NX * (ey + NY * ez)) * EST + (lx + 5 * (ly + 5 * lz));
-                    fx[sl] = ldnt(E.f + q);
-                    fy[sl] = ldnt(E.f + NP + q);
+                    const f2v t = *(cgf2v *)(E.f + 2 * (size_t)q);
+                    fx[sl] = t.x;
+                    fy[sl] = t.y;
                     fz[sl] = ldnt(E.f + 2 * NP + q);
                 }
             }
@@ -775,6 +771,17 @@ __device__ __forceinline__ void node_update(MT &Ms, ET *__restrict__ evs, const 
     stnt(v + g, tg * vx);
     stnt(v + Ms.NGB + g, tg * vy);
     stnt(v + 2 * Ms.NGB + g, tg * vz);
+}
+
+// `gt`: the lane's index among the section's n_events * nodeblocks_per_event * 64
+// node lanes (32-bit: n_events * padded nodes stays below 2^31, checked on the host).
+template <typename MT, typename ET>
+__device__ __forceinline__ void node_update(MT &Ms, ET *__restrict__ evs, const int nodeblocks_per_event,
+                                            const int n_events, const int gt) {
+    const int nbt = gt >> 6;
+    const int ev = nbt / nodeblocks_per_event;
+    if (ev >= n_events) return;
+    node_block(Ms, evs[ev], nbt - ev * nodeblocks_per_event, gt & 63);
 }
 
 __global__ __launch_bounds__(NODE_MAX_THREADS) void k_node(const MeshDev *__restrict__ mesh,
@@ -993,7 +1000,7 @@ struct ses3d_hip_context : StepRunner {
     ses3d::MeshSetup ms;
     bool model_set = false, initialised = false;
     // device mesh data
-    float *lam = nullptr, *mu = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *Q = nullptr,
+    float *lm = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *Q = nullptr,
           *rhoinv = nullptr, *MM = nullptr;
     float *Z = nullptr, *S = nullptr, *COT = nullptr, *LUT = nullptr, *w5 = nullptr;
     unsigned short *IX = nullptr, *IY = nullptr, *IZ = nullptr;
@@ -1743,8 +1750,7 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
                 cfg->zmax, cfg->zmax_global, cfg->dt);
     int rc = 0;
     const long long NP = c->NP, NGB = c->NGB;
-    rc = rc ? rc : dalloc(c, &c->lam, NP);
-    rc = rc ? rc : dalloc(c, &c->mu, NP);
+    rc = rc ? rc : dalloc(c, &c->lm, 2 * NP);
     rc = rc ? rc : dalloc(c, &c->A, NP);
     rc = rc ? rc : dalloc(c, &c->B, NP);
     rc = rc ? rc : dalloc(c, &c->C, NP);
@@ -1788,12 +1794,20 @@ int ses3d_hip_set_model(ses3d_hip_context *c, const float *rhoinv, const float *
     if (!c || !rhoinv || !mu || !lambda || !A || !B || !C) return fail(SES3D_HIP_EINVAL, "null model field");
     if (c->cfg.is_diss && !Q) return fail(SES3D_HIP_EINVAL, "is_diss=1 needs Q");
     std::vector<float> tmp(c->NP);
-    const float *src[7] = {rhoinv, mu, lambda, A, B, C, Q};
-    float *dst[7] = {c->rhoinv, c->mu, c->lam, c->A, c->B, c->C, c->Q};
-    for (int f = 0; f < 7; ++f) {
+    const float *src[5] = {rhoinv, A, B, C, Q};
+    float *dst[5] = {c->rhoinv, c->A, c->B, c->C, c->Q};
+    for (int f = 0; f < 5; ++f) {
         if (!dst[f] || !src[f]) continue;
         ref_to_em(c, src[f], tmp.data());
         TRY(upload(c, dst[f], tmp.data(), sizeof(float) * c->NP));
+    }
+    {
+        std::vector<float> pairs(2 * c->NP);
+        ref_to_em(c, lambda, tmp.data());
+        for (long long i = 0; i < c->NP; ++i) pairs[2 * i] = tmp[i];
+        ref_to_em(c, mu, tmp.data());
+        for (long long i = 0; i < c->NP; ++i) pairs[2 * i + 1] = tmp[i];
+        TRY(upload(c, c->lm, pairs.data(), sizeof(float) * 2 * c->NP));
     }
     c->aniso = false;
     for (long long i = 0; i < c->NPTS && !c->aniso; ++i)
@@ -1893,7 +1907,7 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     std::memcpy(M.tab + 100, ms.DUM1, sizeof(ms.DUM1));
     std::memcpy(M.tab + 125, ms.DUM2, sizeof(ms.DUM2));
     std::memcpy(M.tab + 150, ms.DUM3, sizeof(ms.DUM3));
-    M.lam = c->lam; M.mu = c->mu; M.A = c->A; M.B = c->B; M.C = c->C; M.Q = c->Q; M.rhoinv = c->rhoinv;
+    M.lm = c->lm; M.A = c->A; M.B = c->B; M.C = c->C; M.Q = c->Q; M.rhoinv = c->rhoinv;
     M.MM = c->MM; M.Z = c->Z; M.S = c->S; M.COT = c->COT; M.IX = c->IX; M.IY = c->IY; M.IZ = c->IZ;
     M.LUT = c->LUT;
     M.PMLX = c->PMLX; M.PMLY = c->PMLY; M.PMLZ = c->PMLZ;
@@ -2228,8 +2242,9 @@ int ses3d_hip_get_kernels(ses3d_hip_context *c, int32_t ev, double *g_rho, doubl
     }
     if (g_Q && c->cfg.is_diss) {
         // normalisation at it==nt (SOURCE/ses3d_grad_Q.f90:151-153): 2 mu K / Q
-        std::vector<float> mu(NP), q(NP), mur(c->NPTS), qr(c->NPTS);
-        TRY(download(c, mu.data(), c->mu, sizeof(float) * NP));
+        std::vector<float> lm(2 * NP), mu(NP), q(NP), mur(c->NPTS), qr(c->NPTS);
+        TRY(download(c, lm.data(), c->lm, sizeof(float) * 2 * NP));
+        for (long long i = 0; i < NP; ++i) mu[i] = lm[2 * i + 1];
         TRY(download(c, q.data(), c->Q, sizeof(float) * NP));
         em_to_ref(c, mu.data(), mur.data());
         em_to_ref(c, q.data(), qr.data());
