@@ -155,6 +155,21 @@ int ses3d_hip_get_history(ses3d_hip_context *ctx, int32_t ev, int32_t step, floa
 int ses3d_hip_put_history(ses3d_hip_context *ctx, int32_t ev, int32_t step, const float *in9);
 int ses3d_hip_get_saving_vector(ses3d_hip_context *ctx, int32_t *sv);
 
+/* Polynomial degree `deg` (0..4) of the stored forward fields: the `deg`
+ * argument of ses3d_store / ses3d_restore (SOURCE/ses3d_output.f90:5-267),
+ * which the reference's call sites fix at lpd=4 (SOURCE/ses3d_main.f90:148-158,
+ * SOURCE/ses3d_grad.f90:33-45).  With deg < 4 each element's field is stored
+ * as its values at the (deg+1)^3 degree-deg knots (ses3d_output.f90:36-130)
+ * and interpolated back to the GLL nodes where the kernels read it
+ * (:242-265), with the reference's float32 arithmetic and summation order:
+ * (deg+1)^3/125 of the memory and traffic of the stored fields, lossy.  A
+ * stored step then holds, per field, the (0:nx,0:ny,0:nz,0:deg,0:deg,0:deg)
+ * array ses3d_store writes; get/put_history use that layout.  Set it before
+ * init, equal in the forward and the adjoint context; changing it releases
+ * the stored steps.  history_size gives the floats of one stored step. */
+int ses3d_hip_set_history_degree(ses3d_hip_context *ctx, int32_t deg);
+int ses3d_hip_history_size(ses3d_hip_context *ctx, int64_t *n);
+
 /* Gradient without disk history: for an adjoint_flag==2 context, take the
  * forward wave field from HBM checkpoints.  ses3d_hip_forward_checkpointed
  * runs the forward problem of context fwd (adjoint_flag==1) for all nt

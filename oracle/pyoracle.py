@@ -68,6 +68,13 @@ def _lib(precision: int):
     lib.o_get_receivers.argtypes = [P, dp]
     lib.o_get_history.argtypes = [P, C.c_int, fp]
     lib.o_get_history.restype = C.c_int
+    rp = fp if precision == 32 else dp
+    lib.o_compress.argtypes = [C.c_int, C.c_int64, rp, rp]
+    lib.o_restore.argtypes = [C.c_int, C.c_int64, rp, rp]
+    lib.o_set_history_degree.argtypes = [P, C.c_int]
+    lib.o_set_history_degree.restype = C.c_int
+    lib.o_history_size.argtypes = [P]
+    lib.o_history_size.restype = C.c_int64
     lib.o_saving_vector.argtypes = [P, C.c_int]
     lib.o_lgll.restype = C.c_double
     lib.o_lgll.argtypes = [C.c_int, C.c_double]
@@ -182,12 +189,23 @@ class Oracle:
         self.lib.o_get_receivers(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out[:6 * nr].reshape(nr, 6)
 
+    def set_history_degree(self, deg: int):
+        """Degree of the stored forward fields (ses3d_store's `deg`); before the first step."""
+        if self.lib.o_set_history_degree(self.h, int(deg)) != 0:
+            raise ValueError(f"history degree {deg}")
+        self.hdeg = int(deg)
+
     def history(self, step: int) -> np.ndarray:
-        n = int(np.prod(self.shape_local))
+        """Stored step: (9,) + shape_local, or with degree d < 4 the (9, nx+1,
+        ny+1, nz+1, d+1, d+1, d+1) values ses3d_store writes."""
+        d = getattr(self, "hdeg", 4)
+        shape = self.shape_local if d == 4 else self.shape_local[:3] + (d + 1,) * 3
+        n = int(np.prod(shape))
+        assert 9 * n == self.lib.o_history_size(self.h)
         out = np.zeros(9 * n, np.float32)
         if self.lib.o_get_history(self.h, step, out.ctypes.data_as(C.POINTER(C.c_float))) != 0:
             raise KeyError(step)
-        return np.stack([out[k * n:(k + 1) * n].reshape(self.shape_local, order="F") for k in range(9)])
+        return np.stack([out[k * n:(k + 1) * n].reshape(shape, order="F") for k in range(9)])
 
     def saving_vector(self) -> np.ndarray:
         return np.array([self.lib.o_saving_vector(self.h, it) for it in range(1, self.p.nt + 1)])
@@ -199,6 +217,31 @@ def lgll(i: int, x: float, precision: int = 32) -> float:
 
 def dlgll(i: int, j: int, precision: int = 32) -> float:
     return _lib(precision).o_dlgll(i, j)
+
+
+def _real(precision):
+    return (np.float32, C.c_float) if precision == 32 else (np.float64, C.c_double)
+
+
+def compress(deg: int, field: np.ndarray, precision: int = 32) -> np.ndarray:
+    """ses3d_store's degree-deg values of one field: (nx+1, ny+1, nz+1, 5, 5, 5)
+    -> (nx+1, ny+1, nz+1, deg+1, deg+1, deg+1), Fortran order."""
+    dt, ct = _real(precision)
+    f = np.asarray(field, dt).ravel(order="F")
+    ne = f.size // 125
+    c = np.zeros(ne * (deg + 1) ** 3, dt)
+    _lib(precision).o_compress(deg, ne, f.ctypes.data_as(C.POINTER(ct)), c.ctypes.data_as(C.POINTER(ct)))
+    return c.reshape(field.shape[:3] + (deg + 1,) * 3, order="F")
+
+
+def restore(deg: int, values: np.ndarray, precision: int = 32) -> np.ndarray:
+    """ses3d_restore: degree-deg values back to the 125 GLL nodes."""
+    dt, ct = _real(precision)
+    c = np.asarray(values, dt).ravel(order="F")
+    ne = c.size // (deg + 1) ** 3
+    f = np.zeros(ne * 125, dt)
+    _lib(precision).o_restore(deg, ne, c.ctypes.data_as(C.POINTER(ct)), f.ctypes.data_as(C.POINTER(ct)))
+    return f.reshape(values.shape[:3] + (5, 5, 5), order="F")
 
 
 # ---------------------------------------------------------------------------

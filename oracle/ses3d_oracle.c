@@ -23,6 +23,8 @@
  *                     :445-572 (adjoint sources); ses3d_input.f90:405-437
  *   o_init_pml        SOURCE/ses3d_init.f90:578-709
  *   lgll / dlgll      SOURCE/ses3d_miscellaneous.f90:377-481, :492-613
+ *   o_store / o_grad  degree-compressed stored fields (hdeg < 4):
+ *                     ses3d_store / ses3d_restore, ses3d_output.f90:36-130, :242-265
  *   o_step            SOURCE/ses3d_evolution.f90:7-373 with
  *                     make_moment_source (ses3d_miscellaneous.f90:97-150) and
  *                     add_single_force (ses3d_miscellaneous.f90:11-94),
@@ -102,7 +104,9 @@ typedef struct o_sim {
     /* forward history (adjoint_flag 1 writes, 2 reads) */
     int n_hist;
     int *hist_step;
-    float **hist;   /* 9 x NP float32 per stored step (as written to disk) */
+    float **hist;   /* per stored step as written to disk: 9 x NP float32, or with
+                       hdeg < 4 the 9 degree-hdeg fields (9 x NE x (hdeg+1)^3) */
+    int hdeg;       /* polynomial degree of the stored fields (4 = uncompressed) */
     int *saving_vector;
     /* kernels */
     real *grad_rho, *grad_cp, *grad_csh, *grad_csv, *grad_Q_mu;
@@ -124,6 +128,24 @@ static real lgll(int i, real xv) {
     real v = (real)1.0;
     for (int k = 0; k <= LPD; ++k)
         if (k != i) v = (v * (xv - KN[k])) / (KN[i] - KN[k]);
+    return v;
+}
+
+/* lgll(n, i, x) of SOURCE/ses3d_miscellaneous.f90:377-481 for the degrees
+ * 0..3 that the degree-compressed store uses (knot literals of :396-416). */
+static real knot_n(int n, int i) {
+    static const real K[5][5] = {{(real)0.0},
+                                 {(real)-1.0, (real)1.0},
+                                 {(real)-1.0, (real)0.0, (real)1.0},
+                                 {(real)-1.0, (real)-0.4472135954999579, (real)0.4472135954999579, (real)1.0},
+                                 {(real)-1.0, (real)-0.6546536707079772, (real)0.0, (real)0.6546536707079772, (real)1.0}};
+    return K[n][i];
+}
+static real lgll_n(int n, int i, real xv) {
+    real v = (real)1.0;
+    if (n == 0) return v;
+    for (int k = 0; k <= n; ++k)
+        if (k != i) v = (v * (xv - knot_n(n, k))) / (knot_n(n, i) - knot_n(n, k));
     return v;
 }
 
@@ -175,7 +197,22 @@ o_sim *o_create(const o_params *p) {
     o->sin_x = alloc0(o->NX * NN); o->cos_x = alloc0(o->NX * NN);
     o->so = alloc0(p->nt + 1);
     o->saving_vector = (int *)calloc((size_t)p->nt + 2, sizeof(int));
+    o->hdeg = LPD;
     return o;
+}
+
+/* Degree of the stored forward fields (ses3d_store/ses3d_restore argument
+ * `deg`, SOURCE/ses3d_output.f90:5-267); 4 = lpd stores them unchanged. */
+int o_set_history_degree(o_sim *o, int deg) {
+    if (deg < 0 || deg > LPD || o->n_hist) return -1;
+    o->hdeg = deg;
+    return 0;
+}
+
+/* floats per stored step */
+int64_t o_history_size(o_sim *o) {
+    const int64_t D = o->hdeg + 1;
+    return o->hdeg < LPD ? 9 * o->NE * D * D * D : 9 * o->NP;
 }
 
 void o_destroy(o_sim *o) {
@@ -753,15 +790,76 @@ static void o_record(o_sim *o, int it) {
     }
 }
 
+/* ses3d_store with deg < lpd (ses3d_output.f90:36-130) for one field in
+ * reference layout (NE elements fastest, then the 125 nodes): the degree-4
+ * interpolant sampled at the degree-deg knots, c[e + NE*(i + D(j + D k))] =
+ * sum over ip (x), iq (y), ir (z), in that order from 0, of
+ * f(e,ip,iq,ir) * ((L4_ip(k_i) L4_iq(k_j)) L4_ir(k_k)). */
+void o_compress(int deg, int64_t NE, const real *f, real *c) {
+    const int D = deg + 1;
+    real Lc[NN][4];
+    for (int ip = 0; ip < NN; ++ip)
+        for (int i = 0; i < D; ++i) Lc[ip][i] = lgll(ip, knot_n(deg, i));
+    for (int k = 0; k < D; ++k) for (int j = 0; j < D; ++j) for (int i = 0; i < D; ++i)
+        for (int64_t e = 0; e < NE; ++e) {
+            real acc = (real)0.0;
+            for (int ip = 0; ip < NN; ++ip) for (int iq = 0; iq < NN; ++iq) for (int ir = 0; ir < NN; ++ir) {
+                const real w = (Lc[ip][i] * Lc[iq][j]) * Lc[ir][k];
+                acc = acc + f[e + NE * (ip + NN * (iq + NN * ir))] * w;
+            }
+            c[e + NE * (i + D * (j + D * k))] = acc;
+        }
+}
+
+/* ses3d_restore with deg < lpd (ses3d_output.f90:242-265): back to the GLL
+ * nodes, f(e,l,m,n) = sum over il (x), im (y), in (z), in that order from 0,
+ * of c(e,il,im,in) * ((Ld_il(x_l) Ld_im(x_m)) Ld_in(x_n)). */
+void o_restore(int deg, int64_t NE, const real *c, real *f) {
+    const int D = deg + 1;
+    real Ld[4][NN];
+    for (int il = 0; il < D; ++il)
+        for (int l = 0; l < NN; ++l) Ld[il][l] = lgll_n(deg, il, KN[l]);
+    for (int n = 0; n < NN; ++n) for (int m = 0; m < NN; ++m) for (int l = 0; l < NN; ++l)
+        for (int64_t e = 0; e < NE; ++e) {
+            real acc = (real)0.0;
+            for (int il = 0; il < D; ++il) for (int im = 0; im < D; ++im) for (int in = 0; in < D; ++in) {
+                const real w = (Ld[il][l] * Ld[im][m]) * Ld[in][n];
+                acc = acc + c[e + NE * (il + D * (im + D * in))] * w;
+            }
+            f[e + NE * (l + NN * (m + NN * n))] = acc;
+        }
+}
+
 /* forward store, ses3d_main.f90:142-166: v (pre-update), d, symmetric strain */
 static void o_store(o_sim *o, int it) {
     int h = o->n_hist++;
     o->hist = (float **)realloc(o->hist, sizeof(float *) * o->n_hist);
     o->hist_step = (int *)realloc(o->hist_step, sizeof(int) * o->n_hist);
     o->hist_step[h] = it;
+    int64_t NP = o->NP;
+    if (o->hdeg < LPD) {
+        const int D = o->hdeg + 1;
+        const int64_t NE = o->NE, NC = (int64_t)D * D * D;
+        float *b = (float *)malloc(sizeof(float) * 9 * (size_t)(NE * NC));
+        o->hist[h] = b;
+        real *f = alloc0(NP), *cf = alloc0(NE * NC);
+        for (int c = 0; c < 9; ++c) {
+            for (int64_t q = 0; q < NP; ++q)
+                f[q] = c == 0 ? o->vx[q] : c == 1 ? o->vy[q] : c == 2 ? o->vz[q]
+                     : c == 3 ? o->du[EXX][q] : c == 4 ? o->du[EYY][q] : c == 5 ? o->du[EZZ][q]
+                     : c == 6 ? (o->du[EXY][q] + o->du[EYX][q]) / (real)2
+                     : c == 7 ? (o->du[EXZ][q] + o->du[EZX][q]) / (real)2
+                     : (o->du[EYZ][q] + o->du[EZY][q]) / (real)2;
+            o_compress(o->hdeg, NE, f, cf);
+            for (int64_t i = 0; i < NE * NC; ++i) b[c * NE * NC + i] = (float)cf[i];
+        }
+        free(f);
+        free(cf);
+        o->saving_vector[it] = it;
+        return;
+    }
     float *b = (float *)malloc(sizeof(float) * 9 * (size_t)o->NP);
     o->hist[h] = b;
-    int64_t NP = o->NP;
     for (int64_t q = 0; q < NP; ++q) {
         b[0 * NP + q] = (float)o->vx[q]; b[1 * NP + q] = (float)o->vy[q]; b[2 * NP + q] = (float)o->vz[q];
         b[3 * NP + q] = (float)o->du[EXX][q]; b[4 * NP + q] = (float)o->du[EYY][q]; b[5 * NP + q] = (float)o->du[EZZ][q];
@@ -781,12 +879,25 @@ static void o_grad(o_sim *o, o_sim *fwd, int it) {
     if (h < 0) return;
     const float *b = fwd->hist[h];
     int64_t NP = o->NP;
+    real *dec = NULL;
+    if (fwd->hdeg < LPD) {
+        const int D = fwd->hdeg + 1;
+        const int64_t NE = o->NE, NC = (int64_t)D * D * D;
+        real *cf = alloc0(NE * NC);
+        dec = alloc0(9 * NP);
+        for (int c = 0; c < 9; ++c) {
+            for (int64_t i = 0; i < NE * NC; ++i) cf[i] = (real)b[c * NE * NC + i];
+            o_restore(fwd->hdeg, NE, cf, dec + c * NP);
+        }
+        free(cf);
+    }
+#define FW(c) (dec ? dec[(int64_t)(c) * NP + q] : (real)b[(int64_t)(c) * NP + q])
     const real sdt = (real)o->p.samp_ad * o->dt, Jac = o->Jac;
     #pragma omp parallel for schedule(static)
     for (int64_t q = 0; q < NP; ++q) {
-        real vxf = (real)b[q], vyf = (real)b[NP + q], vzf = (real)b[2 * NP + q];
-        real exxf = (real)b[3 * NP + q], eyyf = (real)b[4 * NP + q], ezzf = (real)b[5 * NP + q];
-        real exyf = (real)b[6 * NP + q], exzf = (real)b[7 * NP + q], eyzf = (real)b[8 * NP + q];
+        real vxf = FW(0), vyf = FW(1), vzf = FW(2);
+        real exxf = FW(3), eyyf = FW(4), ezzf = FW(5);
+        real exyf = FW(6), exzf = FW(7), eyzf = FW(8);
         real exxa = o->du[EXX][q], eyya = o->du[EYY][q], ezza = o->du[EZZ][q];
         real exya = (real)0.5 * (o->du[EXY][q] + o->du[EYX][q]);
         real exza = (real)0.5 * (o->du[EXZ][q] + o->du[EZX][q]);
@@ -813,6 +924,8 @@ static void o_grad(o_sim *o, o_sim *fwd, int it) {
             }
         }
     }
+#undef FW
+    free(dec);
 }
 
 /* One iteration of the reference time loop (ses3d_main.f90:95-178).
@@ -908,11 +1021,13 @@ void o_get_receivers(o_sim *o, double *out) {
     }
 }
 
-/* Forward history access for a stored step (9 x NP float32, order vx vy vz exx eyy ezz exy exz eyz). */
+/* Forward history access for a stored step, order vx vy vz exx eyy ezz exy
+ * exz eyz: 9 x NP float32, or with hdeg < 4 per field the (0:nx,0:ny,0:nz,
+ * 0:hdeg,0:hdeg,0:hdeg) values ses3d_store writes (o_history_size floats). */
 int o_get_history(o_sim *o, int step, float *out) {
     for (int t = 0; t < o->n_hist; ++t)
         if (o->hist_step[t] == step) {
-            memcpy(out, o->hist[t], sizeof(float) * 9 * (size_t)o->NP);
+            memcpy(out, o->hist[t], sizeof(float) * (size_t)o_history_size(o));
             return 0;
         }
     return -1;

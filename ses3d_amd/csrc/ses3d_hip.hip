@@ -50,10 +50,13 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 #ifndef SES3D_EPB
-#define SES3D_EPB 2
+#define SES3D_EPB 1
 #endif
+// minimum waves per SIMD the forward element kernels are compiled for (5 fits
+// them in 96 VGPRs without spilling; the kernels that store or read the
+// forward fields keep the compiler's default)
 #ifndef SES3D_WAVES_PER_EU
-#define SES3D_WAVES_PER_EU 0
+#define SES3D_WAVES_PER_EU 5
 #endif
 // Launch schedule: elements per element-kernel workgroup (1, 2 or 4; 128
 // lanes each, 125 active), lanes per node-kernel workgroup (128, 256 or 512:
@@ -91,7 +94,8 @@ struct MeshDev {
     int is_diss;
     float dt, Jac, sum_D_p;
     float tau_p[3], Lk[3];
-    float tab[7 * 25];         // D1 D2 D3 WDL DUM1 DUM2 DUM3
+    float tab[7 * 25 + 40];    // D1 D2 D3 WDL DUM1 DUM2 DUM3, then the stored-field
+                               // degree tables HLC (5x4) and HLD (4x5)
     const float *lm;   // 2*NP: (lambda, mu) pairs
     const float *A, *B, *C, *Q, *rhoinv, *MM;
     const float *Z, *S, *COT;
@@ -103,6 +107,10 @@ struct MeshDev {
     // checked at init to give the IEEE quotient bit for bit
     float rtau_p[3], r3;
     int fast_tau, fast_3;
+    // stored forward fields: degree hdeg (4: unchanged), hnc = (hdeg+1)^3
+    // values per element and field, hslot floats per sampled step
+    int hdeg, hnc;
+    long long hslot;
 };
 
 struct EventDev {
@@ -277,10 +285,14 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
     const bool act = e < NE;
     const int ndc = nd < 125 ? nd : 124;
 
-    __shared__ float tabs[7 * 25];
+    __shared__ float tabs[7 * 25 + 40];
+    // stored fields of degree < 4: the step's 9 fields of each element (MODE 1,
+    // to be compressed) or its stored values (MODE 2, to be interpolated)
+    __shared__ float hv[MODE ? EPB : 1][MODE ? 9 * TPE : 1];
     __shared__ float sv[EPB][3][TPE];
     __shared__ float ss[EPB][9][TPE];
-    for (int k = t; k < 7 * 25; k += EPB * TPE) tabs[k] = Ms.tab[k];
+    for (int k = t; k < 7 * 25 + 40; k += EPB * TPE) tabs[k] = Ms.tab[k];
+    const bool hcomp = MODE != 0 && Ms.hdeg < 4;   // uniform
     const float *D1 = tabs, *D2 = tabs + 25, *D3 = tabs + 50, *WDL = tabs + 75;
     const float *DUM1 = tabs + 100, *DUM2 = tabs + 125, *DUM3 = tabs + 150;
 
@@ -328,6 +340,18 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             A = ldo(Ms.A, pb);
             B = ldo(Ms.B, pb);
             C = ldo(Ms.C, pb);
+        }
+        if (MODE == 2 && hcomp) {
+            // the element's stored values of the paired forward step, [field][node]
+            const int sl = ldg(E.slot + it);
+            if (sl >= 0) {
+                const int NC = Ms.hnc;
+                const float *hs = E.hist + (size_t)sl * Ms.hslot + (size_t)e * NC;
+                for (int t2 = nd; t2 < 9 * NC; t2 += TPE) {
+                    const int c = t2 / NC;
+                    hv[sub][t2] = ldg(hs + (size_t)c * NE * NC + (t2 - c * NC));
+                }
+            }
         }
     }
     sv[sub][0][nd] = vx;
@@ -515,7 +539,17 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             st9_inner(s, NP, pb2, s9);
         }
 
-        if (MODE == 1) {
+        if (MODE == 1 && hcomp) {
+            // fields to compress after the barrier (ses3d_main.f90:142-166)
+            if (ldg(E.slot + it) >= 0) {
+                float *hw = hv[sub];
+                hw[0 * TPE + nd] = vx; hw[1 * TPE + nd] = vy; hw[2 * TPE + nd] = vz;
+                hw[3 * TPE + nd] = dxux; hw[4 * TPE + nd] = dyuy; hw[5 * TPE + nd] = dzuz;
+                hw[6 * TPE + nd] = (dxuy + dyux) / 2.0f;
+                hw[7 * TPE + nd] = (dxuz + dzux) / 2.0f;
+                hw[8 * TPE + nd] = (dyuz + dzuy) / 2.0f;
+            }
+        } else if (MODE == 1) {
             // forward-field store (ses3d_main.f90:142-166)
             const int sl = ldg(E.slot + it);
             if (sl >= 0) {
@@ -532,10 +566,34 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             // increments in the reference's order, accumulated in float64
             const int sl = ldg(E.slot + it);
             if (sl >= 0) {
-                const float *h = E.hist + (size_t)sl * 9 * NP;
-                const float vxf = h[p], vyf = h[NP + p], vzf = h[2 * NP + p];
-                const float exxf = h[3 * NP + p], eyyf = h[4 * NP + p], ezzf = h[5 * NP + p];
-                const float exyf = h[6 * NP + p], exzf = h[7 * NP + p], eyzf = h[8 * NP + p];
+                float f9[9];
+                if (hcomp) {
+                    // ses3d_restore with deg < lpd (ses3d_output.f90:242-265): the
+                    // stored values interpolated to this node, summed in the order
+                    // il (x), im (y), in (z) from 0
+                    const int D = Ms.hdeg + 1, NC = Ms.hnc;
+                    const float *HLD = tabs + 7 * 25 + 20;
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) f9[c] = 0.f;
+#pragma unroll 1
+                    for (int il = 0; il < D; ++il)
+#pragma unroll 1
+                        for (int im = 0; im < D; ++im)
+#pragma unroll 1
+                            for (int in = 0; in < D; ++in) {
+                                const float w = (HLD[il * 5 + l] * HLD[im * 5 + m]) * HLD[in * 5 + n];
+                                const int idx = il + D * (im + D * in);
+#pragma unroll
+                                for (int c = 0; c < 9; ++c) f9[c] = f9[c] + hv[sub][c * NC + idx] * w;
+                            }
+                } else {
+                    const float *h = E.hist + (size_t)sl * 9 * NP;
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) f9[c] = h[c * NP + p];
+                }
+                const float vxf = f9[0], vyf = f9[1], vzf = f9[2];
+                const float exxf = f9[3], eyyf = f9[4], ezzf = f9[5];
+                const float exyf = f9[6], exzf = f9[7], eyzf = f9[8];
                 const float exya = 0.5f * (dxuy + dyux), exza = 0.5f * (dxuz + dzux), eyza = 0.5f * (dyuz + dzuy);
                 const float tra = (dxux + dyuy) + dzuz, trf = (exxf + eyyf) + ezzf;
                 const float ede = ((((dxux * exxf + dyuy * eyyf) + dzuz * ezzf) + (2.0f * exya) * exyf) +
@@ -573,6 +631,35 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
     }
     __syncthreads();
 
+    if (MODE == 1 && hcomp && act) {
+        // ses3d_store with deg < lpd (ses3d_output.f90:36-130): the degree-4
+        // interpolant of each field sampled at the degree-deg knots, each value
+        // summed over the 125 nodes in the order ip (x), iq (y), ir (z) from 0
+        const int sl = ldg(E.slot + it);
+        if (sl >= 0) {
+            const int D = Ms.hdeg + 1, NC = Ms.hnc;
+            const float *HLC = tabs + 7 * 25;
+            float *hd = E.hist + (size_t)sl * Ms.hslot;
+#pragma unroll 1
+            for (int t2 = nd; t2 < 9 * NC; t2 += TPE) {
+                const int c = t2 / NC, idx = t2 - c * NC;
+                const int i = idx % D, j = (idx / D) % D, k = idx / (D * D);
+                const float *f = hv[sub] + c * TPE;
+                float acc = 0.f;
+#pragma unroll 1
+                for (int ip = 0; ip < 5; ++ip)
+#pragma unroll 1
+                    for (int iq = 0; iq < 5; ++iq) {
+                        const float wxy = HLC[ip * 4 + i] * HLC[iq * 4 + j];
+#pragma unroll
+                        for (int ir = 0; ir < 5; ++ir)
+                            acc = acc + f[ip + 5 * (iq + 5 * ir)] * (wxy * HLC[ir * 4 + k]);
+                    }
+                hd[((size_t)c * NE + e) * NC + idx] = acc;
+            }
+        }
+    }
+
     if (act) {
         // weak-form stress divergence (:237-270); outside the layers the second
         // member of each stress pair is read from the first
@@ -598,9 +685,7 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
 // One section: the step number lives in HBM so a captured step graph replays unchanged.
 template <int DISS, int MODE, int ANISO, int EPB>
 __global__ __launch_bounds__(EPB * TPE)
-#if SES3D_WAVES_PER_EU
-__attribute__((amdgpu_waves_per_eu(SES3D_WAVES_PER_EU, 8)))
-#endif
+__attribute__((amdgpu_waves_per_eu(MODE == 0 && SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1, 8)))
 void k_element(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs, const int *itp,
                int blocks_per_event, int remap) {
     const int blk = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
@@ -635,9 +720,7 @@ __device__ __forceinline__ int section_of(cSectionDev *secs, int nsec, int b, bo
 
 template <int DISS, int MODE, int ANISO, int EPB>
 __global__ __launch_bounds__(EPB * TPE)
-#if SES3D_WAVES_PER_EU
-__attribute__((amdgpu_waves_per_eu(SES3D_WAVES_PER_EU, 8)))
-#endif
+__attribute__((amdgpu_waves_per_eu(MODE == 0 && SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1, 8)))
 void k_element_group(const SectionDev *__restrict__ secs, int nsec, int remap) {
     const int b = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     cSectionDev *cs = (cSectionDev *)secs;
@@ -915,7 +998,7 @@ struct GraphEntry {
 // Launch schedule of the element and node kernels.
 struct Schedule {
     int epb = DEFAULT_EPB;                    // elements per element-kernel workgroup: 1, 2, 4
-    int node_threads = DEFAULT_NODE_THREADS;  // lanes per node-kernel workgroup: 128, 256, 512
+    int node_threads = NODE_MAX_THREADS;      // lanes per node-kernel workgroup: 128, 256, 512
     int remap_elem = 0, remap_node = 0;       // XCD-aware workgroup order (measured defaults)
 };
 
@@ -999,6 +1082,7 @@ struct ses3d_hip_context : StepRunner {
     int generation = 0;               // counts inits (kernel arguments may change)
     ses3d::MeshSetup ms;
     bool model_set = false, initialised = false;
+    int hdeg = 4;                        // degree of the stored forward fields (4: unchanged)
     // device mesh data
     float *lm = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *Q = nullptr,
           *rhoinv = nullptr, *MM = nullptr;
@@ -1168,6 +1252,12 @@ int zero(ses3d_hip_context *c, void *p, size_t bytes) {
 
 int elem_blocks(const ses3d_hip_context *c, int epb) { return (int)((c->NE + epb - 1) / epb); }
 long long node_blocks(const ses3d_hip_context *c) { return (long long)c->NBX * c->NBY * c->NBZ; }
+// floats of one sampled step of stored forward fields: 9 fields of NP values,
+// or of NE*(hdeg+1)^3 values when stored at a lower degree
+long long hist_slot_floats(const ses3d_hip_context *c) {
+    const long long D = c->hdeg + 1;
+    return c->hdeg < 4 ? 9 * c->NE * D * D * D : 9 * c->NP;
+}
 
 // One element-kernel launch: a single section (secs == nullptr) or a group.
 struct ElemLaunch {
@@ -1498,7 +1588,7 @@ int build_slots(ses3d_hip_context *c, EventHost &E, bool allocate_history) {
         if ((it % sa) == 0 || it == 1 || it == nt) E.slot_h[it] = n++;
     E.n_slots = n;
     if (!E.slot_d) TRY(dalloc(c, &E.slot_d, nt + 1));
-    if (allocate_history && !E.hist_own) TRY(dalloc(c, &E.hist_own, (size_t)n * 9 * c->NP));
+    if (allocate_history && !E.hist_own) TRY(dalloc(c, &E.hist_own, (size_t)n * hist_slot_floats(c)));
     if (allocate_history) E.hist = E.hist_own;
     return 0;
 }
@@ -1907,6 +1997,10 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     std::memcpy(M.tab + 100, ms.DUM1, sizeof(ms.DUM1));
     std::memcpy(M.tab + 125, ms.DUM2, sizeof(ms.DUM2));
     std::memcpy(M.tab + 150, ms.DUM3, sizeof(ms.DUM3));
+    ses3d::stored_degree_tables(c->hdeg, M.tab + 175, M.tab + 195);
+    M.hdeg = c->hdeg;
+    M.hnc = (c->hdeg + 1) * (c->hdeg + 1) * (c->hdeg + 1);
+    M.hslot = hist_slot_floats(c);
     M.lm = c->lm; M.A = c->A; M.B = c->B; M.C = c->C; M.Q = c->Q; M.rhoinv = c->rhoinv;
     M.MM = c->MM; M.Z = c->Z; M.S = c->S; M.COT = c->COT; M.IX = c->IX; M.IY = c->IY; M.IZ = c->IZ;
     M.LUT = c->LUT;
@@ -2261,9 +2355,20 @@ int ses3d_hip_get_history(ses3d_hip_context *c, int32_t ev, int32_t step_, float
     if (step_ < 1 || step_ > c->cfg.nt || E.slot_h.empty() || E.slot_h[step_] < 0)
         return fail(SES3D_HIP_EINVAL, "step is not a stored forward step");
     const long long NP = c->NP;
+    const float *src = E.hist_own + (long long)E.slot_h[step_] * hist_slot_floats(c);
+    if (c->hdeg < 4) {
+        // device [field][element][node] -> reference (0:nx,0:ny,0:nz,0:deg,0:deg,0:deg) per field
+        const long long NE = c->NE, NC = hist_slot_floats(c) / (9 * NE);
+        std::vector<float> tmp(9 * NE * NC);
+        TRY(download(c, tmp.data(), src, sizeof(float) * tmp.size()));
+        for (long long k = 0; k < 9; ++k)
+            for (long long e = 0; e < NE; ++e)
+                for (long long i = 0; i < NC; ++i) out9[(k * NC + i) * NE + e] = tmp[(k * NE + e) * NC + i];
+        return 0;
+    }
     std::vector<float> tmp(NP);
     for (int k = 0; k < 9; ++k) {
-        TRY(download(c, tmp.data(), E.hist_own + ((long long)E.slot_h[step_] * 9 + k) * NP, sizeof(float) * NP));
+        TRY(download(c, tmp.data(), src + k * NP, sizeof(float) * NP));
         em_to_ref(c, tmp.data(), out9 + k * c->NPTS);
     }
     return 0;
@@ -2279,11 +2384,44 @@ int ses3d_hip_put_history(ses3d_hip_context *c, int32_t ev, int32_t step_, const
     if (step_ < 1 || step_ > c->cfg.nt || E.slot_h.empty() || E.slot_h[step_] < 0)
         return fail(SES3D_HIP_EINVAL, "step is not a sampled forward step");
     const long long NP = c->NP;
+    float *dst = E.hist_own + (long long)E.slot_h[step_] * hist_slot_floats(c);
+    if (c->hdeg < 4) {
+        const long long NE = c->NE, NC = hist_slot_floats(c) / (9 * NE);
+        std::vector<float> tmp(9 * NE * NC);
+        for (long long k = 0; k < 9; ++k)
+            for (long long e = 0; e < NE; ++e)
+                for (long long i = 0; i < NC; ++i) tmp[(k * NE + e) * NC + i] = in9[(k * NC + i) * NE + e];
+        TRY(upload(c, dst, tmp.data(), sizeof(float) * tmp.size()));
+        return 0;
+    }
     std::vector<float> tmp(NP);
     for (int k = 0; k < 9; ++k) {
         ref_to_em(c, in9 + k * c->NPTS, tmp.data());
-        TRY(upload(c, E.hist_own + ((long long)E.slot_h[step_] * 9 + k) * NP, tmp.data(), sizeof(float) * NP));
+        TRY(upload(c, dst + k * NP, tmp.data(), sizeof(float) * NP));
     }
+    return 0;
+}
+
+int ses3d_hip_set_history_degree(ses3d_hip_context *c, int32_t deg) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (deg < 0 || deg > 4) return fail(SES3D_HIP_EINVAL, "stored-field degree must be 0..4");
+    if (deg == c->hdeg) return 0;
+    // buffers of the other layout are released; init rebuilds the tables
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &E : c->evh) {
+        if (E.hist_own) dfree(c, E.hist_own);
+        if (E.seg) dfree(c, E.seg);
+        E.hist_own = E.seg = E.hist = nullptr;
+        E.seg_slots = 0;
+    }
+    c->hdeg = deg;
+    c->initialised = false;
+    return 0;
+}
+
+int ses3d_hip_history_size(ses3d_hip_context *c, int64_t *n) {
+    if (!c || !n) return fail(SES3D_HIP_EINVAL, "null argument");
+    *n = c->hdeg < 4 ? hist_slot_floats(c) : 9 * c->NPTS;   // host (reference-layout) floats
     return 0;
 }
 
@@ -2339,6 +2477,7 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
     if (!f->ckpt_ready) return fail(SES3D_HIP_ESTATE, "run forward_checkpointed on the forward context first");
     if (a->cfg.nt != f->cfg.nt || a->NE != f->NE || a->n_events != f->n_events || a->cfg.samp_ad != f->cfg.samp_ad)
         return fail(SES3D_HIP_EINVAL, "forward and adjoint contexts differ in mesh, nt, samp_ad or event count");
+    if (a->hdeg != f->hdeg) return fail(SES3D_HIP_EINVAL, "forward and adjoint contexts store fields of different degree");
     const int nt = f->cfg.nt, k = f->ckpt_k;
     const int nseg = (nt + k - 1) / k;
     const size_t NP = (size_t)f->NP, NGB = (size_t)f->NGB;
@@ -2359,7 +2498,7 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
                 HIP_TRY(hipStreamSynchronize(f->stream));
                 dfree(f, E.seg);
             }
-            TRY(dalloc(f, &E.seg, (size_t)max_slots * 9 * NP));
+            TRY(dalloc(f, &E.seg, (size_t)max_slots * hist_slot_floats(f)));
             E.seg_slots = max_slots;
         }
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {

@@ -20,6 +20,36 @@ float lgll(int i, float x) {
     return v;
 }
 
+// The reference's knots for Lagrange degree n = 0..4
+// (SOURCE/ses3d_miscellaneous.f90:396-424, also ses3d_output.f90:41-79).
+static float knot_n(int n, int i) {
+    static const float K[5][5] = {{0.0f},
+                                  {-1.0f, 1.0f},
+                                  {-1.0f, 0.0f, 1.0f},
+                                  {-1.0f, -0.4472135954999579f, 0.4472135954999579f, 1.0f},
+                                  {-1.0f, -0.6546536707079772f, 0.0f, 0.6546536707079772f, 1.0f}};
+    return K[n][i];
+}
+
+// Lagrange basis i of degree n at x (SOURCE/ses3d_miscellaneous.f90:377-481).
+float lgll_n(int n, int i, float x) {
+    float v = 1.0f;
+    if (n == 0) return v;
+    for (int k = 0; k <= n; ++k)
+        if (k != i) v = (v * (x - knot_n(n, k))) / (knot_n(n, i) - knot_n(n, k));
+    return v;
+}
+
+void stored_degree_tables(int deg, float *hlc, float *hld) {
+    for (int k = 0; k < 40; ++k) (k < 20 ? hlc[k] : hld[k - 20]) = 0.0f;
+    if (deg < 0 || deg >= 4) return;
+    for (int p = 0; p < NN; ++p)
+        for (int i = 0; i <= deg; ++i) {
+            hlc[p * 4 + i] = lgll(p, knot_n(deg, i));      // lgll(lpd, ip, cknots(i)), ses3d_output.f90:119
+            hld[i * 5 + p] = lgll_n(deg, i, KN[p]);        // lgll(deg, il, knots(i)), ses3d_output.f90:254
+        }
+}
+
 // L_i'(knot_j) (SOURCE/ses3d_miscellaneous.f90:575-605).
 float dlgll(int i, int j) {
     float y;

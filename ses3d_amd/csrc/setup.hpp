@@ -51,6 +51,14 @@ struct PointLoc {
 
 float lgll(int i, float x);
 float dlgll(int i, int j);
+// Lagrange basis i of degree n (0..4) through the reference's knots for
+// that degree (SOURCE/ses3d_miscellaneous.f90:377-481).
+float lgll_n(int n, int i, float x);
+// Tables of the degree-deg stored forward fields (deg < 4; ses3d_store /
+// ses3d_restore, SOURCE/ses3d_output.f90:36-130, :242-265):
+// hlc[p*4+i] = lgll(4, p, knot_deg(i)), hld[i*5+p] = lgll(deg, i, knot_4(p)).
+// Both are 20 floats; unused entries are 0.
+void stored_degree_tables(int deg, float *hlc, float *hld);
 
 SourceLoc locate_source(const MeshSetup &m, float xxs, float yys, float zzs);
 // Moment-tensor weights: md[c*125+node] = MOM_c * delta(node), c in

@@ -75,6 +75,8 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_get_history": ([P, i32, i32, fp], C.c_int),
         "ses3d_hip_put_history": ([P, i32, i32, fp], C.c_int),
         "ses3d_hip_get_saving_vector": ([P, ip], C.c_int),
+        "ses3d_hip_set_history_degree": ([P, i32], C.c_int),
+        "ses3d_hip_history_size": ([P, C.POINTER(C.c_int64)], C.c_int),
         "ses3d_hip_forward_checkpointed": ([P, i32], C.c_int),
         "ses3d_hip_adjoint_checkpointed": ([P, P], C.c_int),
         "ses3d_hip_memory_bytes": ([P, C.POINTER(C.c_int64)], C.c_int),
@@ -276,13 +278,36 @@ class Solver:
         _check(self.lib.ses3d_hip_get_kernels(self.h, ev, *ptrs))
         return {n: a.reshape(self.shape_local, order="F") for n, a in outs.items()}
 
+    def set_history_degree(self, deg: int):
+        """Polynomial degree (0..4) of the stored forward fields, as the `deg`
+        argument of SOURCE/ses3d_output.f90:5-267 (the reference passes 4):
+        with deg < 4 a stored step keeps each element's field at the (deg+1)^3
+        degree-deg knots and the kernels read it interpolated back (lossy,
+        (deg+1)^3/125 of the memory).  Before init; the same in the forward
+        and the adjoint context."""
+        _check(self.lib.ses3d_hip_set_history_degree(self.h, int(deg)))
+        self.hdeg = int(deg)
+
+    def _history_shape(self):
+        d = getattr(self, "hdeg", 4)
+        return self.shape_local if d == 4 else self.shape_local[:3] + (d + 1,) * 3
+
     def history(self, step: int, ev: int = 0) -> np.ndarray:
-        out = np.zeros(9 * self.n_points, np.float32)
+        """Stored forward step: (9,) + shape_local, or with degree d < 4 the
+        (9, nx+1, ny+1, nz+1, d+1, d+1, d+1) values ses3d_store writes."""
+        shape = self._history_shape()
+        n = int(np.prod(shape))
+        size = C.c_int64()
+        _check(self.lib.ses3d_hip_history_size(self.h, C.byref(size)))
+        assert size.value == 9 * n
+        out = np.zeros(9 * n, np.float32)
         _check(self.lib.ses3d_hip_get_history(self.h, ev, step, _fptr(out)))
-        n = self.n_points
-        return np.stack([out[k * n:(k + 1) * n].reshape(self.shape_local, order="F") for k in range(9)])
+        return np.stack([out[k * n:(k + 1) * n].reshape(shape, order="F") for k in range(9)])
 
     def put_history(self, step: int, fields9: np.ndarray, ev: int = 0):
+        shape = self._history_shape()
+        if tuple(np.shape(fields9)) != (9,) + tuple(shape):
+            raise Ses3dError(f"put_history: expected shape {(9,) + tuple(shape)}, got {np.shape(fields9)}")
         buf = np.concatenate([_ref_f32(fields9[k]) for k in range(9)])
         _check(self.lib.ses3d_hip_put_history(self.h, ev, step, _fptr(buf)))
 

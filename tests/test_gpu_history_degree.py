@@ -1,0 +1,136 @@
+"""Forward fields stored at a lower polynomial degree on the GPU: the `deg`
+argument of ses3d_store / ses3d_restore (SOURCE/ses3d_output.f90:5-267),
+which the reference's call sites fix at lpd=4.
+
+* The stored values (each element's field at the (deg+1)^3 degree-deg knots)
+  equal the float32 oracle's bit for bit, for every stored step; the oracle's
+  compression and interpolation are pinned by the polynomial and float64
+  tests of tests/test_oracle.py.
+* The kernels from the stored-history path equal those of the HBM-checkpoint
+  path exactly, and agree with the float32 oracle's (which interpolates the
+  stored values back as ses3d_restore does) to relative L2 <= 1e-5, the bar of
+  the full-degree adjoint test (float64 accumulation on the GPU).
+* Storage at degree 0 or 1 is visibly lossy: its kernels differ from the
+  full-degree kernels by far more than that bar.
+"""
+import numpy as np
+import pytest
+
+from tests.cases import make_case, oracle_for
+from tests.test_gpu_adjoint import _adjoint_sources, _rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(run, af, deg):
+    from ses3d_amd.solver import Solver
+    s = Solver.from_run(run, adjoint_flag=af)
+    s.set_history_degree(deg)
+    s.set_model(run.model)
+    return s
+
+
+def _stored_path(run, e, deg, loc, rows):
+    f = _ctx(run, 1, deg)
+    f.load_event(0, e)
+    f.init()
+    f.run()
+    a = _ctx(run, 2, deg)
+    a.set_adjoint_sources(0, loc, rows)
+    a.init()
+    steps = [int(st) for st in np.flatnonzero(f.saving_vector()) + 1]
+    hist = {st: f.history(st) for st in steps}
+    for st in steps:
+        a.put_history(st, hist[st])
+    a.run()
+    K = a.kernels()
+    f.close()
+    a.close()
+    return hist, K
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_stored_degree_matches_oracle(tmp_path, deg):
+    from oracle import pyoracle as PO
+    run = make_case(str(tmp_path), nel=(5, 4, 3), nt=30, is_diss=True, adjoint_flag=1, samp_ad=3, seed=51 + deg)
+    e = run.events[0]
+    nt = e.nt
+    print(f"stored-field degree {deg}", flush=True)
+    # forward seismograms do not depend on the degree; adjoint sources from them
+    f0 = _ctx(run, 1, 4)
+    f0.load_event(0, e)
+    f0.init()
+    f0.run()
+    loc, rows = _adjoint_sources(e, f0.seismograms())
+    f0.close()
+    hist, K1 = _stored_path(run, e, deg, loc, rows)
+    # stored values vs the oracle, bit for bit
+    of = oracle_for(run, e, adjoint_flag=1)
+    of.set_history_degree(deg)
+    of.run(1, nt)
+    D = deg + 1
+    for st, h in hist.items():
+        assert h.shape == (9, run.nx + 1, run.ny + 1, run.nz + 1, D, D, D)
+        np.testing.assert_array_equal(h, of.history(st), err_msg=f"stored step {st}")
+    # checkpoint path: same kernels exactly
+    f2 = _ctx(run, 1, deg)
+    f2.load_event(0, e)
+    f2.init()
+    f2.forward_checkpointed(7)
+    a2 = _ctx(run, 2, deg)
+    a2.set_adjoint_sources(0, loc, rows)
+    a2.init()
+    a2.adjoint_checkpointed(f2)
+    K2 = a2.kernels()
+    for g in K1:
+        np.testing.assert_array_equal(K2[g], K1[g], err_msg=g)
+    f2.close()
+    a2.close()
+    # kernels vs the float32 oracle interpolating the stored values back
+    oa = PO.Oracle(nx=run.nx, ny=run.ny, nz=run.nz, xmin=run.xmin, xmax=run.xmax, ymin=run.ymin,
+                   ymax=run.ymax, zmin=run.zmin, zmax=run.zmax, zmax_global=run.zmax_global, nt=nt,
+                   dt=e.dt, is_diss=run.is_diss, adjoint_flag=2, samp_ad=run.samp_ad, pml=2,
+                   tau_p=run.tau_p, D_p=run.D_p)
+    oa.set_model(run.model)
+    oa.set_adjoint_sources(loc, rows)
+    oa.init()
+    oa.run(1, nt, fwd=of)
+    for g in K1:
+        ref = oa.get(g)
+        assert np.linalg.norm(ref) > 0, g
+        rel = _rel(K1[g], ref)
+        print(f"  {g} vs oracle32: rel L2 {rel:.2e}", flush=True)
+        assert rel <= 1e-5, (g, rel)
+    if deg <= 1:
+        _, K4 = _stored_path(run, e, 4, loc, rows)
+        lossy = _rel(K1["grad_csh"], K4["grad_csh"])
+        print(f"  grad_csh at degree {deg} vs degree 4: rel L2 {lossy:.2e}", flush=True)
+        assert lossy > 1e-3
+
+
+def test_stored_degree_rejects_bad_input(tmp_path):
+    from ses3d_amd.solver import Ses3dError
+    run = make_case(str(tmp_path), nel=(3, 3, 3), nt=8, is_diss=True, adjoint_flag=1, samp_ad=2, seed=3)
+    f = _ctx(run, 1, 2)
+    with pytest.raises(Ses3dError, match="degree"):
+        f.set_history_degree(5)
+    with pytest.raises(Ses3dError, match="degree"):
+        f.set_history_degree(-1)
+    f.load_event(0, run.events[0])
+    f.init()
+    f.run()
+    a = _ctx(run, 2, 2)
+    with pytest.raises(Ses3dError, match="shape"):
+        a.put_history(1, np.zeros((9,) + f.shape_local, np.float32))
+    a.close()
+    b = _ctx(run, 2, 3)
+    b.set_adjoint_sources(0, run.events[0].rec32[:1], np.zeros((1, run.events[0].nt, 3), np.float32))
+    b.init()
+    f2 = _ctx(run, 1, 2)
+    f2.load_event(0, run.events[0])
+    f2.init()
+    f2.forward_checkpointed(4)
+    with pytest.raises(Ses3dError, match="degree"):
+        b.adjoint_checkpointed(f2)
+    for s in (f, b, f2):
+        s.close()

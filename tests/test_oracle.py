@@ -177,3 +177,70 @@ def test_rank_boxes_tile_the_section():
             dz = 1000e3 / 12
             assert abs(b.zmax - (6371e3 - sl[2].start * dz)) < 1e-6
         assert (cover == 1).all(), ranks
+
+
+def _lagrange(knots, i, x):
+    """float64 Lagrange basis i through `knots` at x (textbook product form)."""
+    v = 1.0
+    for k, xk in enumerate(knots):
+        if k != i:
+            v *= (x - xk) / (knots[i] - xk)
+    return v
+
+
+GLL4 = np.array([-1.0, -np.sqrt(3.0 / 7.0), 0.0, np.sqrt(3.0 / 7.0), 1.0])
+# knots of the stored degree: 0..3 (Lobatto points of that degree)
+KNOTS = {0: np.array([0.0]), 1: np.array([-1.0, 1.0]), 2: np.array([-1.0, 0.0, 1.0]),
+         3: np.array([-1.0, -1.0 / np.sqrt(5.0), 1.0 / np.sqrt(5.0), 1.0])}
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_stored_field_degree_pins(oracle_lib, deg):
+    """Degree-deg storage of the forward fields (ses3d_store / ses3d_restore,
+    SOURCE/ses3d_output.f90:36-130, :242-265), which the reference's call sites
+    never enable (they pass lpd), is pinned by mathematics:
+    * a field that is, per element, a polynomial of degree <= deg in each
+      coordinate (random, non-symmetric coefficients, so a swapped axis or a
+      wrong knot table fails) is stored as its values at the degree-deg Lobatto
+      knots and restored exactly (to rounding);
+    * for an arbitrary field both maps equal an independent float64 einsum of
+      the Lagrange weights (axis order of the reference's loops)."""
+    from oracle import pyoracle as PO
+    rng = np.random.default_rng(40 + deg)
+    shape = (3, 2, 2)
+    D = deg + 1
+    a = rng.standard_normal(shape + (D, D, D))
+    pw = lambda x: np.array([x ** p for p in range(D)])     # noqa: E731
+    Vg = np.stack([pw(x) for x in GLL4])                     # (5, D)
+    Vk = np.stack([pw(x) for x in KNOTS[deg]])               # (D, D)
+    f = np.einsum("xyzpqr,lp,mq,nr->xyzlmn", a, Vg, Vg, Vg)
+    ck = np.einsum("xyzpqr,ip,jq,kr->xyzijk", a, Vk, Vk, Vk)
+    for prec, tol in ((64, 1e-12), (32, 2e-6)):
+        c = PO.compress(deg, f, prec)
+        np.testing.assert_allclose(c, ck, rtol=0, atol=tol * np.abs(ck).max())
+        r = PO.restore(deg, c, prec)
+        np.testing.assert_allclose(r, f, rtol=0, atol=tol * np.abs(f).max())
+    # arbitrary field against the float64 einsum of the Lagrange weights
+    g = rng.standard_normal(shape + (5, 5, 5))
+    Wc = np.array([[_lagrange(GLL4, p, x) for x in KNOTS[deg]] for p in range(5)])   # (5, D)
+    Wd = np.array([[_lagrange(KNOTS[deg], i, x) if deg else 1.0 for x in GLL4] for i in range(D)])  # (D, 5)
+    cg = np.einsum("xyzpqr,pi,qj,rk->xyzijk", g, Wc, Wc, Wc)
+    rg = np.einsum("xyzijk,il,jm,kn->xyzlmn", cg, Wd, Wd, Wd)
+    for prec, tol in ((64, 1e-12), (32, 2e-6)):
+        c = PO.compress(deg, g, prec)
+        np.testing.assert_allclose(c, cg, rtol=0, atol=tol * np.abs(cg).max())
+        np.testing.assert_allclose(PO.restore(deg, cg, prec), rg, rtol=0, atol=tol * np.abs(rg).max())
+
+
+def test_stored_field_knot_literals():
+    """The degree-3 knot literal the reference spells (ses3d_output.f90:59-60,
+    ses3d_miscellaneous.f90:414-415), 0.4472135954999579 = 1/sqrt(5), rounds
+    to the same float32 directly and through float64 (no double rounding), so
+    the C float literal and the Fortran real literal agree."""
+    from fractions import Fraction
+    for lit in ("0.4472135954999579", "0.6546536707079772"):
+        x = Fraction(lit)
+        f = np.float32(float(x))
+        cands = (np.nextafter(f, np.float32(-1)), f, np.nextafter(f, np.float32(2)))
+        assert min(cands, key=lambda c: abs(Fraction(float(c)) - x)) == f
+    assert abs(float(Fraction("0.4472135954999579")) - 1 / np.sqrt(5)) < 1e-16
