@@ -108,14 +108,16 @@ def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
                        f"({times[1]:.1f} s and {times[0]:.1f} s)")}
 
 
-def gradient_leg(k: int = 100, nt=None, n_events: int = 1) -> dict:
+def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int = 4) -> dict:
     """Seconds per event-gradient on the differentiation workload (48x48x24
     anelastic, dt=0.15): data from the 'true' model (1D + 2% Gaussian blobs),
     gradient of the L2 waveform misfit at the 1D model by the checkpointed
     forward (restart states every k steps in HBM) and the adjoint with kernel
     accumulation, for n_events events solved together in one context (event 1:
     100 surface receivers; the others seeded sources 5-600 km deep with 10-400
-    receivers).  The timed region is forward + adjoint of all events."""
+    receivers).  The timed region is forward + adjoint of all events.
+    history_degree < 4 stores the sampled forward fields at that polynomial
+    degree (ses3d_store's `deg`; lossy, smaller segment buffer)."""
     from dataclasses import replace
     from ses3d_amd import configs, misfit
     from ses3d_amd.solver import Solver
@@ -133,11 +135,13 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1) -> dict:
     f0.close()
     del true
     f = Solver.from_run(cur, adjoint_flag=1, pml=w.pml, nt=nt, n_events=n_events)
+    f.set_history_degree(history_degree)
     f.set_model(cur.model)
     for j in range(n_events):
         f.load_event(j, cur.events[j])
     f.init()
     a = Solver.from_run(cur, adjoint_flag=2, pml=w.pml, nt=nt, n_events=n_events)
+    a.set_history_degree(history_degree)
     a.set_model(cur.model)
     dt = float(cur.events[0].dt)
     taper = misfit.cosine_taper(nt, dt, 40.0)
@@ -159,6 +163,8 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1) -> dict:
            "checkpoint_every": k, "receivers": [int(f.nr(j)) for j in range(n_events)],
            "gll_points": w.n_points, "misfit": chi, "kernels_finite_nonzero": bool(ok),
            "device_GiB": (f.memory_bytes() + a.memory_bytes()) / 2 ** 30}
+    if history_degree < 4:
+        out["history_degree"] = history_degree
     f.close()
     a.close()
     return out
@@ -294,6 +300,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--history-degree", type=int, default=4,
+                    help="polynomial degree of the stored forward fields in the gradient legs (4: unchanged)")
     ap.add_argument("--gradient-events", type=int, default=4,
                     help="events solved together in the batched gradient leg (about 57 GB of HBM each)")
     ap.add_argument("--inversion", action="store_true",
@@ -385,9 +393,9 @@ def main():
     s.close()
     if not args.no_gradient:
         print("# gradient leg", file=sys.stderr, flush=True)
-        out["gradient"] = gradient_leg()
+        out["gradient"] = gradient_leg(history_degree=args.history_degree)
         print("# batched gradient leg", file=sys.stderr, flush=True)
-        gb = gradient_leg(n_events=args.gradient_events)
+        gb = gradient_leg(n_events=args.gradient_events, history_degree=args.history_degree)
         gb["per_event_speedup_over_plain"] = out["gradient"]["s_per_event_gradient"] / gb["s_per_event_gradient"]
         out["gradient_batched"] = gb
     if not args.no_batched:

@@ -52,9 +52,8 @@ int fail(int code, const std::string &msg) {
 #ifndef SES3D_EPB
 #define SES3D_EPB 1
 #endif
-// minimum waves per SIMD the forward element kernels are compiled for (5 fits
-// them in 96 VGPRs without spilling; the kernels that store or read the
-// forward fields keep the compiler's default)
+// minimum waves per SIMD the element kernels are compiled for (5 fits every
+// variant in 96 VGPRs without spilling)
 #ifndef SES3D_WAVES_PER_EU
 #define SES3D_WAVES_PER_EU 5
 #endif
@@ -122,9 +121,10 @@ struct EventDev {
     int src_elem;      // moment-tensor source element or -1
     const float *src_md;   // 9*125 MOM_c*delta
     const float *so;       // nt source time function
-    float *hist;           // sampled forward fields, 9*NP per slot
+    float *hist;           // sampled forward fields, 9*NP per slot: pair planes (vx,vy)
+                           // (vz,exx) (eyy,ezz) (exy,exz), eyz at 8*NP (or degree < 4 values)
     const int *slot;       // [it] -> slot or -1 (nt+1 entries)
-    double *grad;          // 5*NP  rho cp csh csv Q_mu
+    double *grad;          // 5*NP  pair planes (rho,cp) (csh,csv), Q_mu at 4*NP
     float sdt;             // samp_ad*dt
 };
 
@@ -175,6 +175,8 @@ __device__ __forceinline__ void sto(float *base, unsigned boff, float v) { *(gfl
 //   M (18 components, index c*3+k): plane q = (2q, 2q+1).
 // Outside the absorbing layers only planes 0..2 of s are touched.
 typedef float f2v __attribute__((ext_vector_type(2)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) d2v gd2v;
 typedef __attribute__((address_space(1))) const f2v cgf2v;
 typedef __attribute__((address_space(1))) f2v gf2v;
 __device__ __forceinline__ f2v ldo2(const float *base, unsigned boff) {
@@ -554,11 +556,11 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             const int sl = ldg(E.slot + it);
             if (sl >= 0) {
                 float *h = E.hist + (size_t)sl * 9 * NP;
-                h[0 * NP + p] = vx; h[1 * NP + p] = vy; h[2 * NP + p] = vz;
-                h[3 * NP + p] = dxux; h[4 * NP + p] = dyuy; h[5 * NP + p] = dzuz;
-                h[6 * NP + p] = (dxuy + dyux) / 2.0f;
-                h[7 * NP + p] = (dxuz + dzux) / 2.0f;
-                h[8 * NP + p] = (dyuz + dzuy) / 2.0f;
+                sto2(h, pb2, vx, vy);
+                sto2(h + 2 * NP, pb2, vz, dxux);
+                sto2(h + 4 * NP, pb2, dyuy, dzuz);
+                sto2(h + 6 * NP, pb2, (dxuy + dyux) / 2.0f, (dxuz + dzux) / 2.0f);
+                sto(h + 8 * NP, pb, (dyuz + dzuy) / 2.0f);
             }
         }
         if (MODE == 2) {
@@ -589,7 +591,12 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
                 } else {
                     const float *h = E.hist + (size_t)sl * 9 * NP;
 #pragma unroll
-                    for (int c = 0; c < 9; ++c) f9[c] = h[c * NP + p];
+                    for (int q = 0; q < 4; ++q) {
+                        const f2v t = ldo2(h + 2 * q * NP, pb2);
+                        f9[2 * q] = t.x;
+                        f9[2 * q + 1] = t.y;
+                    }
+                    f9[8] = ldo(h + 8 * NP, pb);
                 }
                 const float vxf = f9[0], vyf = f9[1], vzf = f9[2];
                 const float exxf = f9[3], eyyf = f9[4], ezzf = f9[5];
@@ -610,10 +617,16 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
                 const float g_csh = (sdt * (((4.0f * rho) * cs) *
                                             (((ede - tra * trf) - (2.0f * exzf) * exza) - (2.0f * eyzf) * eyza))) / Jac;
                 const float g_csv = (sdt * (((8.0f * rho) * cs) * (exzf * exza + eyzf * eyza))) / Jac;
-                gr[p] = (gr[p] + (double)g_rho1) + (double)g_rho2;
-                gr[NP + p] = gr[NP + p] + (double)g_cp;
-                gr[2 * NP + p] = gr[2 * NP + p] + (double)g_csh;
-                gr[3 * NP + p] = gr[3 * NP + p] + (double)g_csv;
+                {
+                    gd2v *g01 = (gd2v *)(gr + 2 * (size_t)p), *g23 = (gd2v *)(gr + 2 * NP + 2 * (size_t)p);
+                    d2v a = *g01, b = *g23;
+                    a.x = (a.x + (double)g_rho1) + (double)g_rho2;
+                    a.y = a.y + (double)g_cp;
+                    b.x = b.x + (double)g_csh;
+                    b.y = b.y + (double)g_csv;
+                    *g01 = a;
+                    *g23 = b;
+                }
                 if (DISS) {
                     double gq = gr[4 * NP + p];
 #pragma unroll
@@ -685,7 +698,7 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
 // One section: the step number lives in HBM so a captured step graph replays unchanged.
 template <int DISS, int MODE, int ANISO, int EPB>
 __global__ __launch_bounds__(EPB * TPE)
-__attribute__((amdgpu_waves_per_eu(MODE == 0 && SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1, 8)))
 void k_element(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs, const int *itp,
                int blocks_per_event, int remap) {
     const int blk = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
@@ -720,7 +733,7 @@ __device__ __forceinline__ int section_of(cSectionDev *secs, int nsec, int b, bo
 
 template <int DISS, int MODE, int ANISO, int EPB>
 __global__ __launch_bounds__(EPB * TPE)
-__attribute__((amdgpu_waves_per_eu(MODE == 0 && SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1, 8)))
 void k_element_group(const SectionDev *__restrict__ secs, int nsec, int remap) {
     const int b = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     cSectionDev *cs = (cSectionDev *)secs;
@@ -2331,7 +2344,13 @@ int ses3d_hip_get_kernels(ses3d_hip_context *c, int32_t ev, double *g_rho, doubl
     for (int k = 0; k < 5; ++k) {
         if (!outs[k]) continue;
         if (k == 4 && !c->cfg.is_diss) return fail(SES3D_HIP_EINVAL, "grad_Q_mu needs is_diss=1");
-        TRY(download(c, tmp.data(), E.grad + k * NP, sizeof(double) * NP));
+        if (k < 4) {   // pair plane k / 2, member k % 2
+            std::vector<double> pl(2 * NP);
+            TRY(download(c, pl.data(), E.grad + (k / 2) * 2 * NP, sizeof(double) * 2 * NP));
+            for (long long i = 0; i < NP; ++i) tmp[i] = pl[2 * i + (k % 2)];
+        } else {
+            TRY(download(c, tmp.data(), E.grad + 4 * NP, sizeof(double) * NP));
+        }
         em_to_ref(c, tmp.data(), outs[k]);
     }
     if (g_Q && c->cfg.is_diss) {
@@ -2366,9 +2385,11 @@ int ses3d_hip_get_history(ses3d_hip_context *c, int32_t ev, int32_t step_, float
                 for (long long i = 0; i < NC; ++i) out9[(k * NC + i) * NE + e] = tmp[(k * NE + e) * NC + i];
         return 0;
     }
-    std::vector<float> tmp(NP);
+    // pair planes (vx,vy) (vz,exx) (eyy,ezz) (exy,exz), then eyz
+    std::vector<float> all(9 * NP), tmp(NP);
+    TRY(download(c, all.data(), src, sizeof(float) * 9 * NP));
     for (int k = 0; k < 9; ++k) {
-        TRY(download(c, tmp.data(), src + k * NP, sizeof(float) * NP));
+        for (long long i = 0; i < NP; ++i) tmp[i] = k < 8 ? all[(k / 2) * 2 * NP + 2 * i + (k % 2)] : all[8 * NP + i];
         em_to_ref(c, tmp.data(), out9 + k * c->NPTS);
     }
     return 0;
@@ -2394,11 +2415,12 @@ int ses3d_hip_put_history(ses3d_hip_context *c, int32_t ev, int32_t step_, const
         TRY(upload(c, dst, tmp.data(), sizeof(float) * tmp.size()));
         return 0;
     }
-    std::vector<float> tmp(NP);
+    std::vector<float> all(9 * NP, 0.0f), tmp(NP, 0.0f);
     for (int k = 0; k < 9; ++k) {
         ref_to_em(c, in9 + k * c->NPTS, tmp.data());
-        TRY(upload(c, dst + k * NP, tmp.data(), sizeof(float) * NP));
+        for (long long i = 0; i < NP; ++i) (k < 8 ? all[(k / 2) * 2 * NP + 2 * i + (k % 2)] : all[8 * NP + i]) = tmp[i];
     }
+    TRY(upload(c, dst, all.data(), sizeof(float) * 9 * NP));
     return 0;
 }
 
