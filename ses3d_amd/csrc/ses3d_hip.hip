@@ -918,6 +918,16 @@ __global__ void k_verify_div(const float *b, const float *y, int nd, unsigned lo
     }
 }
 
+// One component of an element-major array (point p at src[p * stride]) into
+// the reference layout (element fastest, node slowest): dst[e + NE * nd].
+template <typename T>
+__global__ void k_to_ref(T *__restrict__ dst, const T *__restrict__ src, long long NE, int stride) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NE * 125) return;
+    const long long nd = i / NE, e = i - nd * NE;
+    dst[i] = src[(e * EST + nd) * stride];
+}
+
 __global__ void k_set_step(int *itp, int it) {
     if (threadIdx.x == 0) *itp = it;
 }
@@ -1124,6 +1134,8 @@ struct ses3d_hip_context : StepRunner {
     std::vector<std::pair<void *, size_t>> allocs;
     int64_t bytes = 0;
     unsigned long long *div_buf = nullptr;        // divisors, reciprocals, failure counts
+    float *conv = nullptr;                         // layout-conversion scratch for downloads
+    size_t conv_cap = 0;
 
     int launch_receivers_on(hipStream_t st, bool record);
     int launch_forces_on(hipStream_t st);
@@ -1255,6 +1267,19 @@ int download(ses3d_hip_context *c, void *dst, const void *src, size_t bytes) {
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+// Component of an element-major device array (first point at `comp`, `stride`
+// values between points) to a host array in reference layout, transposed on
+// the GPU into a scratch buffer kept by the context.
+template <typename T>
+int download_ref(ses3d_hip_context *c, T *host, const T *comp, int stride) {
+    const size_t n = (size_t)c->NPTS;
+    TRY(dfit(c, &c->conv, c->conv_cap, (n * sizeof(T) + sizeof(float) - 1) / sizeof(float)));
+    T *d = (T *)c->conv;
+    hipLaunchKernelGGL(k_to_ref<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d, comp, c->NE, stride);
+    HIP_TRY(hipGetLastError());
+    return download(c, host, d, sizeof(T) * n);
 }
 
 int zero(ses3d_hip_context *c, void *p, size_t bytes) {
@@ -2339,28 +2364,19 @@ int ses3d_hip_get_kernels(ses3d_hip_context *c, int32_t ev, double *g_rho, doubl
     EventHost &E = c->evh[ev];
     if (!E.grad) return fail(SES3D_HIP_ESTATE, "kernels exist only for adjoint_flag=2 contexts");
     const long long NP = c->NP;
-    std::vector<double> tmp(NP);
     double *outs[5] = {g_rho, g_cp, g_csh, g_csv, g_Q};
     for (int k = 0; k < 5; ++k) {
         if (!outs[k]) continue;
         if (k == 4 && !c->cfg.is_diss) return fail(SES3D_HIP_EINVAL, "grad_Q_mu needs is_diss=1");
-        if (k < 4) {   // pair plane k / 2, member k % 2
-            std::vector<double> pl(2 * NP);
-            TRY(download(c, pl.data(), E.grad + (k / 2) * 2 * NP, sizeof(double) * 2 * NP));
-            for (long long i = 0; i < NP; ++i) tmp[i] = pl[2 * i + (k % 2)];
-        } else {
-            TRY(download(c, tmp.data(), E.grad + 4 * NP, sizeof(double) * NP));
-        }
-        em_to_ref(c, tmp.data(), outs[k]);
+        // pair planes (rho, cp) (csh, csv), then Q_mu
+        if (k < 4) TRY(download_ref(c, outs[k], E.grad + (k / 2) * 2 * NP + (k % 2), 2));
+        else TRY(download_ref(c, outs[k], E.grad + 4 * NP, 1));
     }
     if (g_Q && c->cfg.is_diss) {
         // normalisation at it==nt (SOURCE/ses3d_grad_Q.f90:151-153): 2 mu K / Q
-        std::vector<float> lm(2 * NP), mu(NP), q(NP), mur(c->NPTS), qr(c->NPTS);
-        TRY(download(c, lm.data(), c->lm, sizeof(float) * 2 * NP));
-        for (long long i = 0; i < NP; ++i) mu[i] = lm[2 * i + 1];
-        TRY(download(c, q.data(), c->Q, sizeof(float) * NP));
-        em_to_ref(c, mu.data(), mur.data());
-        em_to_ref(c, q.data(), qr.data());
+        std::vector<float> mur(c->NPTS), qr(c->NPTS);
+        TRY(download_ref(c, mur.data(), (const float *)c->lm + 1, 2));
+        TRY(download_ref(c, qr.data(), (const float *)c->Q, 1));
         for (long long i = 0; i < c->NPTS; ++i) g_Q[i] = ((2.0 * (double)mur[i]) * g_Q[i]) / (double)qr[i];
     }
     return 0;
@@ -2386,12 +2402,8 @@ int ses3d_hip_get_history(ses3d_hip_context *c, int32_t ev, int32_t step_, float
         return 0;
     }
     // pair planes (vx,vy) (vz,exx) (eyy,ezz) (exy,exz), then eyz
-    std::vector<float> all(9 * NP), tmp(NP);
-    TRY(download(c, all.data(), src, sizeof(float) * 9 * NP));
-    for (int k = 0; k < 9; ++k) {
-        for (long long i = 0; i < NP; ++i) tmp[i] = k < 8 ? all[(k / 2) * 2 * NP + 2 * i + (k % 2)] : all[8 * NP + i];
-        em_to_ref(c, tmp.data(), out9 + k * c->NPTS);
-    }
+    for (int k = 0; k < 9; ++k)
+        TRY(download_ref(c, out9 + k * c->NPTS, k < 8 ? src + (k / 2) * 2 * NP + (k % 2) : src + 8 * NP, k < 8 ? 2 : 1));
     return 0;
 }
 
