@@ -928,6 +928,16 @@ __global__ void k_to_ref(T *__restrict__ dst, const T *__restrict__ src, long lo
     dst[i] = src[(e * EST + nd) * stride];
 }
 
+// Inverse of k_to_ref: reference layout into one component of an
+// element-major array (the 3 padding values of each slot are left as they are).
+template <typename T>
+__global__ void k_from_ref(T *__restrict__ dst, const T *__restrict__ src, long long NE, int stride) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NE * 125) return;
+    const long long nd = i / NE, e = i - nd * NE;
+    dst[(e * EST + nd) * stride] = src[i];
+}
+
 __global__ void k_set_step(int *itp, int it) {
     if (threadIdx.x == 0) *itp = it;
 }
@@ -1233,19 +1243,6 @@ int dfit(ses3d_hip_context *c, T **p, size_t &cap, size_t n) {
     return 0;
 }
 
-// reference layout (element fastest, node slowest) <-> element-major layout
-void ref_to_em(const ses3d_hip_context *c, const float *src, float *dst) {
-    const long long NE = c->NE;
-    for (int nd = 0; nd < 125; ++nd)
-        for (long long e = 0; e < NE; ++e) dst[e * EST + nd] = src[e + NE * nd];
-}
-template <typename T>
-void em_to_ref(const ses3d_hip_context *c, const T *src, T *dst) {
-    const long long NE = c->NE;
-    for (int nd = 0; nd < 125; ++nd)
-        for (long long e = 0; e < NE; ++e) dst[e + NE * nd] = src[e * EST + nd];
-}
-
 long long host_gidx(const ses3d_hip_context *c, int gx, int gy, int gz) {
     return ((((long long)(gz >> 2) * c->NBY + (gy >> 2)) * c->NBX + (gx >> 2)) << 6) +
            (((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
@@ -1280,6 +1277,19 @@ int download_ref(ses3d_hip_context *c, T *host, const T *comp, int stride) {
     hipLaunchKernelGGL(k_to_ref<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d, comp, c->NE, stride);
     HIP_TRY(hipGetLastError());
     return download(c, host, d, sizeof(T) * n);
+}
+
+// Host array in reference layout into one component of an element-major
+// device array (first point at `comp`, `stride` values between points).
+template <typename T>
+int upload_ref(ses3d_hip_context *c, T *comp, const T *host, int stride) {
+    const size_t n = (size_t)c->NPTS;
+    TRY(dfit(c, &c->conv, c->conv_cap, (n * sizeof(T) + sizeof(float) - 1) / sizeof(float)));
+    T *d = (T *)c->conv;
+    TRY(upload(c, d, host, sizeof(T) * n));
+    hipLaunchKernelGGL(k_from_ref<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, comp, d, c->NE, stride);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 int zero(ses3d_hip_context *c, void *p, size_t bytes) {
@@ -1900,6 +1910,14 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
         if (cfg->adjoint_flag == 2) rc = rc ? rc : dalloc(c, &E.grad, 5 * NP);
         E.so.assign(cfg->nt, 0.f);
     }
+    // model arrays start at zero, so the padding values of each element slot
+    // (read only by the tail lanes, which write only padding) are zeros
+    {
+        float *model[6] = {c->lm, c->A, c->B, c->C, c->rhoinv, c->Q};
+        const long long n[6] = {2 * NP, NP, NP, NP, NP, NP};
+        for (int k = 0; k < 6 && !rc; ++k)
+            if (model[k]) rc = zero(c, model[k], sizeof(float) * n[k]);
+    }
     if (rc) {
         ses3d_hip_destroy(c);
         return rc;
@@ -1921,22 +1939,13 @@ int ses3d_hip_set_model(ses3d_hip_context *c, const float *rhoinv, const float *
                         const float *A, const float *B, const float *C, const float *Q) {
     if (!c || !rhoinv || !mu || !lambda || !A || !B || !C) return fail(SES3D_HIP_EINVAL, "null model field");
     if (c->cfg.is_diss && !Q) return fail(SES3D_HIP_EINVAL, "is_diss=1 needs Q");
-    std::vector<float> tmp(c->NP);
     const float *src[5] = {rhoinv, A, B, C, Q};
     float *dst[5] = {c->rhoinv, c->A, c->B, c->C, c->Q};
-    for (int f = 0; f < 5; ++f) {
-        if (!dst[f] || !src[f]) continue;
-        ref_to_em(c, src[f], tmp.data());
-        TRY(upload(c, dst[f], tmp.data(), sizeof(float) * c->NP));
-    }
-    {
-        std::vector<float> pairs(2 * c->NP);
-        ref_to_em(c, lambda, tmp.data());
-        for (long long i = 0; i < c->NP; ++i) pairs[2 * i] = tmp[i];
-        ref_to_em(c, mu, tmp.data());
-        for (long long i = 0; i < c->NP; ++i) pairs[2 * i + 1] = tmp[i];
-        TRY(upload(c, c->lm, pairs.data(), sizeof(float) * 2 * c->NP));
-    }
+    for (int f = 0; f < 5; ++f)
+        if (dst[f] && src[f]) TRY(upload_ref(c, dst[f], src[f], 1));
+    TRY(upload_ref(c, c->lm, lambda, 2));       // (lambda, mu) pairs
+    TRY(upload_ref(c, c->lm + 1, mu, 2));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     c->aniso = false;
     for (long long i = 0; i < c->NPTS && !c->aniso; ++i)
         if (A[i] != 0.0f || B[i] != 0.0f || C[i] != 0.0f) c->aniso = true;
@@ -2270,18 +2279,6 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
     static const char *dn[9] = {"dxux", "dyuy", "dzuz", "dxuy", "dyux", "dxuz", "dzux", "dyuz", "dzuy"};
     static const char *sn[9] = {"sxx", "syy", "szz", "sxy", "syx", "sxz", "szx", "syz", "szy"};
     static const char *mn[6] = {"Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz"};
-    std::vector<float> tmp;
-    // one component of a paired array: download its plane and keep every
-    // stride-th float from the component's offset
-    auto component = [&](std::vector<float> &dst, const float *base, long long off, int stride) -> int {
-        dst.resize(NP);
-        if (stride == 1) return download(c, dst.data(), base + off, sizeof(float) * NP);
-        const long long plane = off & ~1LL;
-        std::vector<float> pl(2 * NP);
-        TRY(download(c, pl.data(), base + plane, sizeof(float) * 2 * NP));
-        for (long long i = 0; i < NP; ++i) dst[i] = pl[2 * i + (off - plane)];
-        return 0;
-    };
     // outside the absorbing layers the second member of each split-stress pair
     // is the first (only the first is stored there)
     static const int pair[9] = {EXX, EYY, EZZ, EXY, EXY, EXZ, EXZ, EYZ, EYZ};
@@ -2290,22 +2287,21 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
             if (c->cfg.adjoint_flag == 0)
                 return fail(SES3D_HIP_EINVAL, "with adjoint_flag=0 the displacement gradient is integrated only "
                                               "inside the absorbing layers (the only place it acts); use adjoint_flag 1 or 2");
-            TRY(component(tmp, E.d, c9_offset(k, NP), c9_stride(k)));
-            em_to_ref(c, tmp.data(), out);
-            return 0;
+            return download_ref(c, out, E.d + c9_offset(k, NP), c9_stride(k));
         }
         if (nm == sn[k]) {
-            TRY(component(tmp, E.s, c9_offset(k, NP), c9_stride(k)));
+            TRY(download_ref(c, out, E.s + c9_offset(k, NP), c9_stride(k)));
             if (pair[k] != k) {
-                std::vector<float> first;
-                TRY(component(first, E.s, c9_offset(pair[k], NP), c9_stride(pair[k])));
+                std::vector<float> first(c->NPTS);
+                TRY(download_ref(c, first.data(), E.s + c9_offset(pair[k], NP), c9_stride(pair[k])));
+                std::vector<char> inner(c->NE);
                 for (long long e = 0; e < c->NE; ++e) {
                     const int i = (int)(e % c->NX), j = (int)((e / c->NX) % c->NY), kk = (int)(e / ((long long)c->NX * c->NY));
-                    if (!(c->pmlx[i] | c->pmly[j] | c->pmlz[kk]))
-                        std::memcpy(&tmp[e * EST], &first[e * EST], sizeof(float) * 125);
+                    inner[e] = !(c->pmlx[i] | c->pmly[j] | c->pmlz[kk]);
                 }
+                for (long long q = 0; q < c->NPTS; ++q)
+                    if (inner[q % c->NE]) out[q] = first[q];
             }
-            em_to_ref(c, tmp.data(), out);
             return 0;
         }
     }
@@ -2314,8 +2310,7 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
             if (!E.M) return fail(SES3D_HIP_EINVAL, "memory variables exist only when is_diss=1");
             for (int mech = 0; mech < 3; ++mech) {
                 const int i = k * 3 + mech;   // pair plane i / 2, member i % 2
-                TRY(component(tmp, E.M, (long long)(i & ~1) * NP + (i & 1), 2));
-                em_to_ref(c, tmp.data(), out + mech * c->NPTS);
+                TRY(download_ref(c, out + mech * c->NPTS, E.M + (long long)(i & ~1) * NP + (i & 1), 2));
             }
             return 0;
         }
@@ -2427,12 +2422,10 @@ int ses3d_hip_put_history(ses3d_hip_context *c, int32_t ev, int32_t step_, const
         TRY(upload(c, dst, tmp.data(), sizeof(float) * tmp.size()));
         return 0;
     }
-    std::vector<float> all(9 * NP, 0.0f), tmp(NP, 0.0f);
-    for (int k = 0; k < 9; ++k) {
-        ref_to_em(c, in9 + k * c->NPTS, tmp.data());
-        for (long long i = 0; i < NP; ++i) (k < 8 ? all[(k / 2) * 2 * NP + 2 * i + (k % 2)] : all[8 * NP + i]) = tmp[i];
-    }
-    TRY(upload(c, dst, all.data(), sizeof(float) * 9 * NP));
+    // pair planes (vx,vy) (vz,exx) (eyy,ezz) (exy,exz), then eyz
+    for (int k = 0; k < 9; ++k)
+        TRY(upload_ref(c, k < 8 ? dst + (k / 2) * 2 * NP + (k % 2) : dst + 8 * NP, in9 + k * c->NPTS, k < 8 ? 2 : 1));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 
