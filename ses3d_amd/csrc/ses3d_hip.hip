@@ -117,7 +117,8 @@ struct EventDev {
     float *d;          // 9*NP  dxux dyuy dzuz dxuy dyux dxuz dzux dyuz dzuy (paired, c9_offset)
     float *s;          // 9*NP  sxx syy szz sxy syx sxz szx syz szy (split stresses; paired, c9_offset)
     float *M;          // 18*NP index c*3+k, c: xx yy zz xy xz yz, k: mechanism (paired)
-    float *f;          // 3*NP  element force: (fx, fy) pair plane, then fz at 2*NP
+    float *f;          // 3*NP  element force: (fx, fy) pair plane, then fz at 2*NP;
+                       // nodes in fperm order within each element slot
     int src_elem;      // moment-tensor source element or -1
     const float *src_md;   // 9*125 MOM_c*delta
     const float *so;       // nt source time function
@@ -188,6 +189,23 @@ __device__ __forceinline__ void sto2(float *base, unsigned boff, float a, float 
     v.y = b;
     *(gf2v *)((gchar *)base + boff) = v;
 }
+// Position of node (l,m,n) in an element's slot of the element force f:
+// first the 4x4x4 nodes with l,m,n < 4 (the node block whose lanes own them in
+// the node kernel) in block-lane order, then the l=4, m=4 and n=4 faces, the
+// three edges and the corner, each in the order its neighbouring block's lanes
+// read it.  A node-kernel wavefront then reads each sharer's contributions as
+// one contiguous run instead of scattered over the 125-node slot.
+__host__ __device__ constexpr int fperm(int l, int m, int n) {
+    return (l < 4 && m < 4 && n < 4) ? l + 4 * (m + 4 * n)
+         : (l == 4 && m < 4 && n < 4) ? 64 + m + 4 * n
+         : (l < 4 && m == 4 && n < 4) ? 80 + l + 4 * n
+         : (l < 4 && m < 4 && n == 4) ? 96 + l + 4 * m
+         : (l == 4 && m == 4 && n < 4) ? 112 + n
+         : (l == 4 && m < 4 && n == 4) ? 116 + m
+         : (l < 4 && m == 4 && n == 4) ? 120 + l
+         : 124;
+}
+
 // float offset of component c's first point in a 9-component array, and
 // the stride between points (2 in a pair plane, 1 for EZY)
 __host__ __device__ constexpr long long c9_offset(int c, long long NP) {
@@ -690,8 +708,10 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             ay = ((ay + ss[sub][czy][pz_] * wz) + ss[sub][EYY][py_] * wy) + ss[sub][EXY][px_] * wx;
             az = ((az + ss[sub][EZZ][pz_] * wz) + ss[sub][EYZ][py_] * wy) + ss[sub][EXZ][px_] * wx;
         }
-        sto2(E.f, pb2, ax, ay);
-        sto(E.f + 2 * NP, pb, az);
+        // tail lanes write the slot's padding, real nodes their fperm position
+        const unsigned fq = 4u * (unsigned)(e * EST + (nd < 125 ? fperm(l, m, n) : nd));
+        sto2(E.f, 2u * fq, ax, ay);
+        sto(E.f + 2 * NP, fq, az);
     }
 }
 
@@ -766,7 +786,7 @@ __global__ void k_forces(const MeshDev *__restrict__ mesh, const EventDev *__res
     if (nd >= 125) return;
     const long long NP = mesh->NP;
     float *f = evs[G.ev].f;
-    const long long p = G.elem * EST + nd;
+    const long long p = G.elem * EST + fperm(nd % 5, (nd / 5) % 5, nd / 25);
     float fx = f[2 * p], fy = f[2 * p + 1], fz = f[2 * NP + p];
     for (int s = G.first; s < G.first + G.count; ++s) {
         const ForceDev F = forces[s];
@@ -823,7 +843,7 @@ __device__ __forceinline__ void node_block(MT &Ms, EV &E, const int nb, const in
                     const int ex = ii ? bx : xe0, lx = ii ? 0 : xl0;
                     const int ey = jj ? by : ye0, ly = jj ? 0 : yl0;
                     const int ez = kk ? bz : ze0, lz = kk ? 0 : zl0;
-                    const int q = (ex + NX * (ey + NY * ez)) * EST + (lx + 5 * (ly + 5 * lz));
+                    const int q = (ex + NX * (ey + NY * ez)) * EST + fperm(lx, ly, lz);
                     const f2v t = *(cgf2v *)(E.f + 2 * (size_t)q);
                     fx[sl] = t.x;
                     fy[sl] = t.y;
