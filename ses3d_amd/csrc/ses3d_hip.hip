@@ -113,7 +113,7 @@ struct MeshDev {
 };
 
 struct EventDev {
-    float *v;          // 3*NGB
+    float *v;          // 3*NGB: (vx, vy) pair plane, then vz at 2*NGB
     float *d;          // 9*NP  dxux dyuy dzuz dxuy dyux dxuz dzux dyuz dzuy (paired, c9_offset)
     float *s;          // 9*NP  sxx syy szz sxy syx sxz szx syz szy (split stresses; paired, c9_offset)
     float *M;          // 18*NP index c*3+k, c: xx yy zz xy xz yz, k: mechanism (paired)
@@ -333,8 +333,9 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
     if (act) {
         // gather v (SOURCE/ses3d_evolution.f90:27-51)
         const long long g = gidx(4 * ei + l, 4 * ej + m, 4 * ek + n, Ms.NBX, Ms.NBY);
-        vx = ldnt(E.v + g);
-        vy = ldnt(E.v + Ms.NGB + g);
+        const f2v vxy = *(cgf2v *)(E.v + 2 * g);
+        vx = vxy.x;
+        vy = vxy.y;
         vz = ldnt(E.v + 2 * Ms.NGB + g);
         if (keep_d) {
             ld9_inner(E.d, NP, pb2, d9);
@@ -880,12 +881,15 @@ __device__ __forceinline__ void node_block(MT &Ms, EV &E, const int nb, const in
     const float dt = Ms.dt;
     float *v = E.v;
     const float ax = Sx / mm, ay = Sy / mm, az = Sz / mm;
-    float vx = ldnt(v + g), vy = ldnt(v + Ms.NGB + g), vz = ldnt(v + 2 * Ms.NGB + g);
+    const f2v vxy = *(cgf2v *)(v + 2 * (size_t)g);
+    float vx = vxy.x, vy = vxy.y, vz = ldnt(v + 2 * Ms.NGB + g);
     vx = vx + dt * (ax - pg * vx);
     vy = vy + dt * (ay - pg * vy);
     vz = vz + dt * (az - pg * vz);
-    stnt(v + g, tg * vx);
-    stnt(v + Ms.NGB + g, tg * vy);
+    f2v o;
+    o.x = tg * vx;
+    o.y = tg * vy;
+    *(gf2v *)(v + 2 * (size_t)g) = o;
     stnt(v + 2 * Ms.NGB + g, tg * vz);
 }
 
@@ -1019,13 +1023,15 @@ __global__ void k_receivers(const MeshDev *__restrict__ mesh, const EventDev *__
     if (r >= nrec) return;
     const RecDev R = recs[r];
     const MeshDev &Ms = *mesh;
-    const float *v = evs[R.ev].v + comp * Ms.NGB;
+    // vx, vy interleaved in one plane, vz in its own
+    const float *v = evs[R.ev].v + (comp < 2 ? comp : 2 * Ms.NGB);
+    const int stride = comp < 2 ? 2 : 1;
     float acc = 0.f;
     for (int i = 0; i < 5; ++i)
         for (int j = 0; j < 5; ++j)
             for (int k = 0; k < 5; ++k) {
                 const long long g = gidx(4 * R.ex + i, 4 * R.ey + j, 4 * R.ez + k, Ms.NBX, Ms.NBY);
-                acc = acc + v[g] * R.w[i + 5 * (j + 5 * k)];
+                acc = acc + v[g * stride] * R.w[i + 5 * (j + 5 * k)];
             }
     R.out[comp * nt + it - 1] = acc;
 }
@@ -2335,17 +2341,19 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
             return 0;
         }
     const int gx = 4 * c->NX + 1, gy = 4 * c->NY + 1, gz = 4 * c->NZ + 1;
-    auto global_out = [&](const float *dev) -> int {
-        std::vector<float> g(c->NGB);
-        TRY(download(c, g.data(), dev, sizeof(float) * c->NGB));
+    // global arrays: one plane (stride 1) or a member of the (vx, vy) pair plane
+    auto global_out = [&](const float *dev, int stride) -> int {
+        std::vector<float> g(c->NGB * stride);
+        TRY(download(c, g.data(), dev, sizeof(float) * c->NGB * stride));
         for (int z = 0; z < gz; ++z)
             for (int y = 0; y < gy; ++y)
-                for (int x = 0; x < gx; ++x) out[x + (long long)gx * (y + (long long)gy * z)] = g[host_gidx(c, x, y, z)];
+                for (int x = 0; x < gx; ++x)
+                    out[x + (long long)gx * (y + (long long)gy * z)] = g[host_gidx(c, x, y, z) * stride];
         return 0;
     };
-    auto local_out = [&](const float *dev) -> int {
-        std::vector<float> g(c->NGB);
-        TRY(download(c, g.data(), dev, sizeof(float) * c->NGB));
+    auto local_out = [&](const float *dev, int stride) -> int {
+        std::vector<float> g(c->NGB * stride);
+        TRY(download(c, g.data(), dev, sizeof(float) * c->NGB * stride));
         const long long NE = c->NE;
         for (int n = 0; n < 5; ++n)
             for (int m = 0; m < 5; ++m)
@@ -2354,22 +2362,22 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
                         for (int j = 0; j < c->NY; ++j)
                             for (int i = 0; i < c->NX; ++i) {
                                 long long e = i + (long long)c->NX * (j + (long long)c->NY * k);
-                                out[e + NE * (l + 5 * (m + 5 * n))] = g[host_gidx(c, 4 * i + l, 4 * j + m, 4 * k + n)];
+                                out[e + NE * (l + 5 * (m + 5 * n))] = g[host_gidx(c, 4 * i + l, 4 * j + m, 4 * k + n) * stride];
                             }
         return 0;
     };
-    if (nm == "vx_global") return global_out(E.v);
-    if (nm == "vy_global") return global_out(E.v + c->NGB);
-    if (nm == "vz_global") return global_out(E.v + 2 * c->NGB);
-    if (nm == "MM_global") return global_out(c->MM);
+    if (nm == "vx_global") return global_out(E.v, 2);
+    if (nm == "vy_global") return global_out(E.v + 1, 2);
+    if (nm == "vz_global") return global_out(E.v + 2 * c->NGB, 1);
+    if (nm == "MM_global") return global_out(c->MM, 1);
     // element-local velocity as the snapshots hold it: the local copy gathered at the
     // start of the last step, i.e. the velocity before that step's update.  Between
     // steps it equals a gather of the current global velocity of the previous step;
     // here it is the gather of the current global velocity (the state after the last
     // completed step), which is what the next step's gather will read.
-    if (nm == "vx") return local_out(E.v);
-    if (nm == "vy") return local_out(E.v + c->NGB);
-    if (nm == "vz") return local_out(E.v + 2 * c->NGB);
+    if (nm == "vx") return local_out(E.v, 2);
+    if (nm == "vy") return local_out(E.v + 1, 2);
+    if (nm == "vz") return local_out(E.v + 2 * c->NGB, 1);
     return fail(SES3D_HIP_EINVAL, "unknown field name " + nm);
 }
 
