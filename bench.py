@@ -300,6 +300,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--checkpoint-every", type=int, default=100,
+                    help="restart states every k steps in the gradient legs")
     ap.add_argument("--history-degree", type=int, default=4,
                     help="polynomial degree of the stored forward fields in the gradient legs (4: unchanged)")
     ap.add_argument("--gradient-events", type=int, default=4,
@@ -393,9 +395,9 @@ def main():
     s.close()
     if not args.no_gradient:
         print("# gradient leg", file=sys.stderr, flush=True)
-        out["gradient"] = gradient_leg(history_degree=args.history_degree)
+        out["gradient"] = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree)
         print("# batched gradient leg", file=sys.stderr, flush=True)
-        gb = gradient_leg(n_events=args.gradient_events, history_degree=args.history_degree)
+        gb = gradient_leg(k=args.checkpoint_every, n_events=args.gradient_events, history_degree=args.history_degree)
         gb["per_event_speedup_over_plain"] = out["gradient"]["s_per_event_gradient"] / gb["s_per_event_gradient"]
         out["gradient_batched"] = gb
     if not args.no_batched:
