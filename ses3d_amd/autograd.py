@@ -63,7 +63,7 @@ class Problem:
 
     def __init__(self, run, data: Sequence[np.ndarray], pml: int = 2, ckpt: int = 100,
                  events_per_batch: Optional[int] = None, q_model: Optional[np.ndarray] = None,
-                 taper_width: float = 0.0, progress=None):
+                 taper_width: float = 0.0, progress=None, history_degree: int = 4):
         self.run = run
         self.data = [np.asarray(d, np.float32) for d in data]
         self.pml = pml
@@ -78,6 +78,9 @@ class Problem:
         nt = run.events[0].nt
         self.taper = MF.cosine_taper(nt, self.dt, taper_width) if taper_width > 0 else None
         self.progress = progress          # optional callable(str), told when a batch is done
+        # polynomial degree of the stored forward fields (4: as the reference
+        # stores them; lower: (d+1)^3/125 of the segment memory, lossy)
+        self.history_degree = history_degree
 
     def fields(self, rho, vp, vsh, vsv, Q=None) -> dict:
         rho, vp, vsh, vsv = (np.asarray(a, np.float64) for a in (rho, vp, vsh, vsv))
@@ -94,6 +97,8 @@ class Problem:
 
     def _solver(self, model, af, n):
         s = Solver.from_run(self.run, adjoint_flag=af, pml=self.pml, n_events=n)
+        if af:
+            s.set_history_degree(self.history_degree)
         s.set_model(model)
         return s
 
