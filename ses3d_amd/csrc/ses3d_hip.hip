@@ -898,7 +898,9 @@ __device__ __forceinline__ void node_block(MT &Ms, EV &E, const int nb, const in
 template <typename MT, typename ET>
 __device__ __forceinline__ void node_update(MT &Ms, ET *__restrict__ evs, const int nodeblocks_per_event,
                                             const int n_events, const int gt) {
-    const int nbt = gt >> 6;
+    // a wavefront owns one node block: the block and event indices are
+    // wavefront-uniform, so their divisions run on the scalar unit
+    const int nbt = __builtin_amdgcn_readfirstlane(gt >> 6);
     const int ev = nbt / nodeblocks_per_event;
     if (ev >= n_events) return;
     node_block(Ms, evs[ev], nbt - ev * nodeblocks_per_event, gt & 63);
