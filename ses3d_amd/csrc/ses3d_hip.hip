@@ -845,10 +845,12 @@ __device__ __forceinline__ void node_block(MT &Ms, EV &E, const int nb, const in
                     const int ey = jj ? by : ye0, ly = jj ? 0 : yl0;
                     const int ez = kk ? bz : ze0, lz = kk ? 0 : zl0;
                     const int q = (ex + NX * (ey + NY * ez)) * EST + fperm(lx, ly, lz);
-                    const f2v t = *(cgf2v *)(E.f + 2 * (size_t)q);
+                    // 32-bit byte offsets from the uniform plane bases (the
+                    // element-local arrays are below 4 GiB, checked on the host)
+                    const f2v t = ldo2(E.f, 8u * (unsigned)q);
                     fx[sl] = t.x;
                     fy[sl] = t.y;
-                    fz[sl] = ldnt(E.f + 2 * NP + q);
+                    fz[sl] = ldo(E.f + 2 * NP, 4u * (unsigned)q);
                 }
             }
     float Sx = 0.f, Sy = 0.f, Sz = 0.f;

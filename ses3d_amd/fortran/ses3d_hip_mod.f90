@@ -137,6 +137,19 @@ module ses3d_hip_mod
        type(c_ptr), value :: ctx
        integer(c_int32_t), intent(out) :: sv(*)
      end function
+
+     ! degree of the stored forward fields (ses3d_store's deg; 4 as the reference)
+     integer(c_int) function ses3d_hip_set_history_degree(ctx, deg) bind(c, name='ses3d_hip_set_history_degree')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: deg
+     end function
+
+     integer(c_int) function ses3d_hip_history_size(ctx, n) bind(c, name='ses3d_hip_history_size')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: n
+     end function
   end interface
 
 contains
