@@ -1168,6 +1168,7 @@ struct ses3d_hip_context : StepRunner {
     std::vector<int> saving_vector;   // 1..nt
     int ckpt_k = 0;
     bool ckpt_ready = false;
+    bool last_seg_ready = false;   // the segment buffer holds the last segment's sampled steps
     // step number in HBM
     int *it_d = nullptr;
     unsigned long long div_bad[4] = {0, 0, 0, 0};   // numerators failing the reciprocal check
@@ -2490,6 +2491,37 @@ int ses3d_hip_get_saving_vector(ses3d_hip_context *c, int32_t *sv) {
     return 0;
 }
 
+// Segment buffer of every event: the sampled steps of the longest segment of
+// k steps.  Table of one segment's sampled steps into it.
+int ensure_segment_buffer(ses3d_hip_context *f, int k) {
+    const int nt = f->cfg.nt, nseg = (nt + k - 1) / k;
+    int max_slots = 0;
+    const EventHost &E0 = f->evh[0];
+    for (int sgi = 0; sgi < nseg; ++sgi) {
+        int n = 0;
+        for (int it = sgi * k + 1; it <= std::min((sgi + 1) * k, nt); ++it) n += E0.slot_h[it] >= 0;
+        max_slots = std::max(max_slots, n);
+    }
+    for (auto &E : f->evh)
+        if (E.seg_slots < max_slots) {
+            if (E.seg) {
+                HIP_TRY(hipStreamSynchronize(f->stream));
+                dfree(f, E.seg);
+            }
+            TRY(dalloc(f, &E.seg, (size_t)max_slots * hist_slot_floats(f)));
+            E.seg_slots = max_slots;
+        }
+    return 0;
+}
+
+std::vector<int> segment_table(const EventHost &E, int nt, int s0, int s1) {
+    std::vector<int> tab(nt + 1, -1);
+    int n = 0;
+    for (int it = s0; it <= s1; ++it)
+        if (E.slot_h[it] >= 0) tab[it] = n++;
+    return tab;
+}
+
 int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
     if (!c) return fail(SES3D_HIP_EINVAL, "null context");
     if (!c->initialised) return fail(SES3D_HIP_ESTATE, "init must precede forward_checkpointed");
@@ -2512,8 +2544,19 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
         TRY(upload_slots(c, E, std::vector<int>(nt + 1, -1)));
     }
     TRY(refresh_events(c));
+    TRY(ensure_segment_buffer(c, k));
     c->ckpt_k = k;
+    c->last_seg_ready = false;
     for (int sgi = 0; sgi < nseg; ++sgi) {
+        if (sgi == nseg - 1) {
+            // the last segment stores its sampled steps right away: the adjoint
+            // starts with it and need not replay it
+            for (auto &E : c->evh) {
+                E.hist = E.seg;
+                TRY(upload_slots(c, E, segment_table(E, nt, sgi * k + 1, nt)));
+            }
+            TRY(refresh_events(c));
+        }
         for (auto &E : c->evh) {
             float *dst = E.ckpt[sgi];
             HIP_TRY(hipMemcpyAsync(dst, E.v, sizeof(float) * 3 * NGB, hipMemcpyDeviceToDevice, c->stream));
@@ -2526,6 +2569,7 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->ckpt_ready = true;
+    c->last_seg_ready = true;
     return 0;
 }
 
@@ -2541,45 +2585,29 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
     const int nseg = (nt + k - 1) / k;
     const size_t NP = (size_t)f->NP, NGB = (size_t)f->NGB;
     const size_t nM = f->cfg.is_diss ? 18 * NP : 0;
-    // segment buffer: the sampled steps of the longest segment
-    int max_slots = 0;
-    {
-        const EventHost &E0 = f->evh[0];
-        for (int sgi = 0; sgi < nseg; ++sgi) {
-            int n = 0;
-            for (int it = sgi * k + 1; it <= std::min((sgi + 1) * k, nt); ++it) n += E0.slot_h[it] >= 0;
-            max_slots = std::max(max_slots, n);
-        }
-    }
-    for (auto &E : f->evh)
-        if (E.seg_slots < max_slots) {
-            if (E.seg) {
-                HIP_TRY(hipStreamSynchronize(f->stream));
-                dfree(f, E.seg);
-            }
-            TRY(dalloc(f, &E.seg, (size_t)max_slots * hist_slot_floats(f)));
-            E.seg_slots = max_slots;
-        }
+    TRY(ensure_segment_buffer(f, k));
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
         const int s0 = sgi * k + 1, s1 = std::min((sgi + 1) * k, nt);
-        // restore the forward state at the start of the segment, replay it into the buffer
-        for (int ev = 0; ev < f->n_events; ++ev) {
-            EventHost &E = f->evh[ev];
-            const float *src = E.ckpt[sgi];
-            HIP_TRY(hipMemcpyAsync(E.v, src, sizeof(float) * 3 * NGB, hipMemcpyDeviceToDevice, f->stream));
-            HIP_TRY(hipMemcpyAsync(E.d, src + 3 * NGB, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, f->stream));
-            HIP_TRY(hipMemcpyAsync(E.s, src + 3 * NGB + 9 * NP, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, f->stream));
-            if (nM) HIP_TRY(hipMemcpyAsync(E.M, src + 3 * NGB + 18 * NP, sizeof(float) * nM, hipMemcpyDeviceToDevice, f->stream));
-            std::vector<int> tab(nt + 1, -1);
-            int n = 0;
-            for (int it = s0; it <= s1; ++it)
-                if (E.slot_h[it] >= 0) tab[it] = n++;
-            E.hist = E.seg;
-            TRY(upload_slots(f, E, tab));
+        if (sgi == nseg - 1 && f->last_seg_ready) {
+            // stored by forward_checkpointed; the replays below overwrite it
+            f->last_seg_ready = false;
+        } else {
+            // restore the forward state at the start of the segment, replay it into the buffer
+            f->last_seg_ready = false;
+            for (int ev = 0; ev < f->n_events; ++ev) {
+                EventHost &E = f->evh[ev];
+                const float *src = E.ckpt[sgi];
+                HIP_TRY(hipMemcpyAsync(E.v, src, sizeof(float) * 3 * NGB, hipMemcpyDeviceToDevice, f->stream));
+                HIP_TRY(hipMemcpyAsync(E.d, src + 3 * NGB, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, f->stream));
+                HIP_TRY(hipMemcpyAsync(E.s, src + 3 * NGB + 9 * NP, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, f->stream));
+                if (nM) HIP_TRY(hipMemcpyAsync(E.M, src + 3 * NGB + 18 * NP, sizeof(float) * nM, hipMemcpyDeviceToDevice, f->stream));
+                E.hist = E.seg;
+                TRY(upload_slots(f, E, segment_table(E, nt, s0, s1)));
+            }
+            TRY(refresh_events(f));
+            TRY(run_steps(f, s0, s1 - s0 + 1, 1, false));
+            HIP_TRY(hipStreamSynchronize(f->stream));
         }
-        TRY(refresh_events(f));
-        TRY(run_steps(f, s0, s1 - s0 + 1, 1, false));
-        HIP_TRY(hipStreamSynchronize(f->stream));
         // adjoint steps paired with this segment's forward steps
         const int a0 = nt + 1 - s1, a1 = nt + 1 - s0;
         for (int ev = 0; ev < a->n_events; ++ev) {
