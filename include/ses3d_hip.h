@@ -174,9 +174,10 @@ int ses3d_hip_history_size(ses3d_hip_context *ctx, int64_t *n);
  * forward wave field from HBM checkpoints.  ses3d_hip_forward_checkpointed
  * runs the forward problem of context fwd (adjoint_flag==1) for all nt
  * steps keeping a restart state every k_ckpt steps in HBM instead of every
- * sampled step; ses3d_hip_adjoint_checkpointed then runs the adjoint of
- * context adj, replaying each forward segment from its checkpoint into an
- * HBM buffer of sampled steps just before the adjoint consumes it.  The
+ * sampled step (and the sampled steps of the last segment);
+ * ses3d_hip_adjoint_checkpointed then runs the adjoint of context adj,
+ * replaying each earlier forward segment from its checkpoint into an HBM
+ * buffer of sampled steps just before the adjoint consumes it.  The
  * kernels equal those of the stored-history path bit for bit.  Replaces the
  * disk round trip of ses3d_store/ses3d_restore (SOURCE/ses3d_output.f90:5-267,
  * SOURCE/ses3d_grad.f90:33-45). */
