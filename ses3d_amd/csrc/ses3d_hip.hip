@@ -1019,25 +1019,35 @@ struct RecDev {
     float *out;         // 3 x nt for this receiver (component-major)
 };
 
-__global__ void k_receivers(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs,
-                            const RecDev *__restrict__ recs, int nrec, int nt, const int *itp) {
+// One workgroup per receiver: lane t forms the product of term t of the
+// reference's sum (i slowest, k fastest) for the three components, then one
+// lane per component adds the 125 products in that order from 0, so every
+// product and every partial sum is rounded as in the sequential loop while
+// the 125 gathers are issued at once.
+__global__ __launch_bounds__(128) void k_receivers(const MeshDev *__restrict__ mesh,
+                                                   const EventDev *__restrict__ evs,
+                                                   const RecDev *__restrict__ recs, int nt, const int *itp) {
+    __shared__ float prod[3][128];
     const int it = *itp;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = t / 3, comp = t - 3 * r;
-    if (r >= nrec) return;
-    const RecDev R = recs[r];
+    const RecDev R = recs[blockIdx.x];
     const MeshDev &Ms = *mesh;
-    // vx, vy interleaved in one plane, vz in its own
-    const float *v = evs[R.ev].v + (comp < 2 ? comp : 2 * Ms.NGB);
-    const int stride = comp < 2 ? 2 : 1;
-    float acc = 0.f;
-    for (int i = 0; i < 5; ++i)
-        for (int j = 0; j < 5; ++j)
-            for (int k = 0; k < 5; ++k) {
-                const long long g = gidx(4 * R.ex + i, 4 * R.ey + j, 4 * R.ez + k, Ms.NBX, Ms.NBY);
-                acc = acc + v[g * stride] * R.w[i + 5 * (j + 5 * k)];
-            }
-    R.out[comp * nt + it - 1] = acc;
+    const int t = threadIdx.x;
+    if (t < 125) {
+        const int i = t / 25, j = (t / 5) % 5, k = t % 5;
+        const long long g = gidx(4 * R.ex + i, 4 * R.ey + j, 4 * R.ez + k, Ms.NBX, Ms.NBY);
+        const float *v = evs[R.ev].v;
+        const f2v vxy = *(cgf2v *)(v + 2 * g);   // (vx, vy) pair plane, then vz
+        const float w = R.w[i + 5 * (j + 5 * k)];
+        prod[0][t] = vxy.x * w;
+        prod[1][t] = vxy.y * w;
+        prod[2][t] = v[2 * Ms.NGB + g] * w;
+    }
+    __syncthreads();
+    if (t < 3) {
+        float acc = 0.f;
+        for (int q = 0; q < 125; ++q) acc = acc + prod[t][q];
+        R.out[t * nt + it - 1] = acc;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1540,9 +1550,7 @@ int autotune_steps(StepRunner *r, int it_begin, int n, int mode, bool record) {
 
 int ses3d_hip_context::launch_receivers_on(hipStream_t st, bool record) {
     if (!(record && nrec_total > 0)) return 0;
-    const int threads = 3 * nrec_total;
-    hipLaunchKernelGGL(k_receivers, dim3((threads + 127) / 128), dim3(128), 0, st, mesh_d, ev_d, recs_d, nrec_total,
-                       cfg.nt, it_d);
+    hipLaunchKernelGGL(k_receivers, dim3(nrec_total), dim3(128), 0, st, mesh_d, ev_d, recs_d, cfg.nt, it_d);
     HIP_TRY(hipGetLastError());
     return 0;
 }
