@@ -170,6 +170,31 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int =
     return out
 
 
+def small_forward_leg(name: str = "elastic_24x24x12", warmup: int = 20) -> dict:
+    """The reference's CPU-runnable case on the GPU: the whole time loop of the
+    elastic 24x24x12 workload (864000 GLL points, 20 receivers, nt=2000), the
+    first `warmup` steps autotuning the launch schedule, the rest timed.  At
+    ~150 MB of state per step it lives in the GPU's caches, so it measures
+    launch and latency costs rather than HBM bandwidth."""
+    from ses3d_amd import configs
+    from ses3d_amd.solver import Solver
+    w = configs.WORKLOADS[name]
+    run = configs.build(w, seed=0)
+    nt = run.events[0].nt
+    s = Solver.from_run(run, adjoint_flag=0, pml=w.pml)
+    s.set_model(run.model)
+    s.load_event(0, run.events[0])
+    s.init()
+    s.autotune(1, warmup)
+    steps = nt - warmup
+    wall = _timed_run(s, warmup + 1, steps)
+    ok = bool(np.isfinite(s.seismograms()).all())
+    s.close()
+    return {"workload": f"plain forward {name} (elastic, {w.n_points} GLL points, nt={nt})",
+            "updates_per_s": w.n_points * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "timed_steps": steps, "seismograms_finite": ok}
+
+
 def _timed_run(x, it0: int, steps: int) -> float:
     x.synchronize()
     t0 = time.perf_counter()
@@ -393,6 +418,8 @@ def main():
         "schedule": sched,
     }
     s.close()
+    print("# elastic 24x24x12 leg", file=sys.stderr, flush=True)
+    out["elastic_24x24x12"] = small_forward_leg()
     if not args.no_gradient:
         print("# gradient leg", file=sys.stderr, flush=True)
         out["gradient"] = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree)
