@@ -325,6 +325,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-elastic", action="store_true", help="skip the elastic 24x24x12 leg")
     ap.add_argument("--checkpoint-every", type=int, default=100,
                     help="restart states every k steps in the gradient legs")
     ap.add_argument("--history-degree", type=int, default=4,
@@ -418,8 +419,9 @@ def main():
         "schedule": sched,
     }
     s.close()
-    print("# elastic 24x24x12 leg", file=sys.stderr, flush=True)
-    out["elastic_24x24x12"] = small_forward_leg()
+    if not args.no_elastic:
+        print("# elastic 24x24x12 leg", file=sys.stderr, flush=True)
+        out["elastic_24x24x12"] = small_forward_leg()
     if not args.no_gradient:
         print("# gradient leg", file=sys.stderr, flush=True)
         out["gradient"] = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree)
