@@ -106,52 +106,68 @@ class Problem:
         """Plain forward solve of every event -> chi (no state kept)."""
         chi = 0.0
         self.last_seismograms = []
-        for evs in self._batches():
-            s = self._solver(model, 0, len(evs))
-            for k, ev in enumerate(evs):
-                s.load_event(k, self.run.events[ev])
-            s.init()
-            s.run()
-            for k, ev in enumerate(evs):
-                u = s.seismograms(k)
-                self.last_seismograms.append(u)
-                chi += MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
-            s.close()
-            if self.progress:
-                self.progress(f"forward solve of events {evs[0]}..{evs[-1]} done")
+        ctx = {}    # one context per batch size, reused by every batch of that size
+        try:
+            for evs in self._batches():
+                n = len(evs)
+                if n not in ctx:
+                    ctx[n] = self._solver(model, 0, n)
+                s = ctx[n]
+                for k, ev in enumerate(evs):
+                    s.load_event(k, self.run.events[ev])
+                s.init()
+                s.run()
+                for k, ev in enumerate(evs):
+                    u = s.seismograms(k)
+                    self.last_seismograms.append(u)
+                    chi += MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
+                if self.progress:
+                    self.progress(f"forward solve of events {evs[0]}..{evs[-1]} done")
+        finally:
+            for s in ctx.values():
+                s.close()
         return chi
 
     def misfit_and_gradient(self, model):
-        """chi and the SES3D kernels summed over events (float64 dict)."""
+        """chi and the SES3D kernels summed over events (float64 dict).  The
+        forward and adjoint contexts of a batch size are created once and
+        reused by every batch of that size, so their checkpoint and segment
+        buffers are allocated once."""
         chi = 0.0
         K = None
         self.last_seismograms = []
-        for evs in self._batches():
-            f = self._solver(model, 1, len(evs))
-            for k, ev in enumerate(evs):
-                f.load_event(k, self.run.events[ev])
-            f.init()
-            f.forward_checkpointed(self.ckpt)
-            a = self._solver(model, 2, len(evs))
-            for k, ev in enumerate(evs):
-                u = f.seismograms(k)
-                self.last_seismograms.append(u)
-                chi += MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
-                rows = MF.l2_adjoint_rows(u, self.data[ev], self.dt, taper=self.taper)
-                a.set_adjoint_sources(k, self.run.events[ev].rec32[f.receiver_index(k)], rows)
-            a.init()
-            a.adjoint_checkpointed(f)
-            for k in range(len(evs)):
-                kk = a.kernels(k)
-                if K is None:
-                    K = {n: v.copy() for n, v in kk.items()}
-                else:
-                    for n in K:
-                        K[n] += kk[n]
-            f.close()
-            a.close()
-            if self.progress:
-                self.progress(f"gradient of events {evs[0]}..{evs[-1]} done")
+        ctx = {}
+        try:
+            for evs in self._batches():
+                n = len(evs)
+                if n not in ctx:
+                    ctx[n] = (self._solver(model, 1, n), self._solver(model, 2, n))
+                f, a = ctx[n]
+                for k, ev in enumerate(evs):
+                    f.load_event(k, self.run.events[ev])
+                f.init()
+                f.forward_checkpointed(self.ckpt)
+                for k, ev in enumerate(evs):
+                    u = f.seismograms(k)
+                    self.last_seismograms.append(u)
+                    chi += MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
+                    rows = MF.l2_adjoint_rows(u, self.data[ev], self.dt, taper=self.taper)
+                    a.set_adjoint_sources(k, self.run.events[ev].rec32[f.receiver_index(k)], rows)
+                a.init()
+                a.adjoint_checkpointed(f)
+                for k in range(n):
+                    kk = a.kernels(k)
+                    if K is None:
+                        K = {name: v.copy() for name, v in kk.items()}
+                    else:
+                        for name in K:
+                            K[name] += kk[name]
+                if self.progress:
+                    self.progress(f"gradient of events {evs[0]}..{evs[-1]} done")
+        finally:
+            for f, a in ctx.values():
+                f.close()
+                a.close()
         return chi, K
 
     def gradient_from_kernels(self, K: dict, model: dict) -> dict:

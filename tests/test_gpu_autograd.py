@@ -127,3 +127,31 @@ def test_steepest_descent_update_lowers_misfit():
     print(f"misfit before {chi_before:.6e} after {chi_after:.6e}", flush=True)
     assert abs(chi_before - float(chi0.detach())) <= 1e-6 * chi_before
     assert chi_after < chi_before
+
+
+def test_context_reuse_across_batches():
+    """Events solved one per batch (the contexts created once and reloaded for
+    every event) give exactly the misfit and kernels of both events solved in
+    one two-event context, and so do plain misfit evaluations."""
+    from ses3d_amd.autograd import Problem
+    from ses3d_amd.solver import Solver
+    w = configs.Workload("reuse", (8, 8, 4), True, 2, 0.25, 200, 16, "blobs", period=25.0, n_events=2)
+    true = configs.build(w, seed=3)
+    cur = configs.build(replace(w, perturbation="none"), seed=3)
+    data = []
+    for k in range(2):
+        s = Solver.from_run(true, adjoint_flag=0, pml=2, n_events=1)
+        s.set_model(true.model)
+        s.load_event(0, true.events[k])
+        s.init()
+        s.run()
+        data.append(s.seismograms())
+        s.close()
+    one = Problem(cur, data, pml=2, ckpt=40, taper_width=20.0, events_per_batch=1)
+    two = Problem(cur, data, pml=2, ckpt=40, taper_width=20.0, events_per_batch=2)
+    c1, K1 = one.misfit_and_gradient(cur.model)
+    c2, K2 = two.misfit_and_gradient(cur.model)
+    assert c1 == c2 and c1 > 0
+    for n in K2:
+        np.testing.assert_array_equal(K1[n], K2[n], err_msg=n)
+    assert one.misfit(cur.model) == two.misfit(cur.model) == c1
