@@ -50,7 +50,8 @@ def quadrature_weights(run) -> np.ndarray:
     X = x[:, None, None, :, None, None]
     Z = z[None, None, :, None, None, :]
     W = w3[None, None, None] * (jac * jac) * Z ** 2 * np.sin(X)
-    return np.ascontiguousarray(np.broadcast_to(W, (box.nx + 1, box.ny + 1, box.nz + 1, 5, 5, 5)))
+    # Fortran order, as the kernels come back, so products with them stream
+    return np.asfortranarray(np.broadcast_to(W, (box.nx + 1, box.ny + 1, box.nz + 1, 5, 5, 5)))
 
 
 class Problem:
@@ -156,9 +157,9 @@ class Problem:
                 a.init()
                 a.adjoint_checkpointed(f)
                 for k in range(n):
-                    kk = a.kernels(k)
+                    kk = a.kernels(k)      # fresh arrays, Fortran order
                     if K is None:
-                        K = {name: v.copy() for name, v in kk.items()}
+                        K = dict(kk)
                     else:
                         for name in K:
                             K[name] += kk[name]
