@@ -2,17 +2,23 @@
 // spectral-element solver (lpd = 4), behind the C-ABI of include/ses3d_hip.h.
 //
 // Data layout in HBM (per section):
-//   element-local fields  f[c][e*128 + nd]   e = i + NX*(j + NY*k), nd = l + 5*(m + 5*n)
-//                                            (125 nodes in a 512-byte aligned slot per element)
-//   global fields         v[c][blocked]      4x4x4 node blocks: node (gx,gy,gz) lives in
-//                                            block (gx>>2, gy>>2, gz>>2) at (gz&3)*16+(gy&3)*4+(gx&3)
+//   element-local fields  slot e*128 + nd    e = i + NX*(j + NY*k), nd = l + 5*(m + 5*n)
+//                                            (125 nodes in a 512-byte aligned slot per element;
+//                                            the element force in node-block order, fperm)
+//                         components stored in pairs (two floats per point side by side) so a
+//                         lane moves two components with one 8-byte access
+//   global fields         v[blocked]         4x4x4 node blocks: node (gx,gy,gz) lives in
+//                                            block (gx>>2, gy>>2, gz>>2) at (gz&3)*16+(gy&3)*4+(gx&3);
+//                                            (vx, vy) as one pair plane, vz alone
 // Per step (SOURCE/ses3d_evolution.f90:7-373):
-//   k_receivers  seismograms from the pre-update velocity (ses3d_output.f90:475-495)
+//   k_receivers  seismograms from the pre-update velocity (ses3d_output.f90:475-495),
+//                one workgroup per receiver
 //   k_element    gather v -> 9 velocity gradients (5-term GLL contractions through LDS)
 //                -> spherical terms -> displacement-gradient update -> absorbing-layer terms
 //                -> stress rates (elastic or 3-mechanism anelastic) -> split-stress update
 //                -> weak-form divergence (LDS) + moment source -> element force;
-//                memory-variable update, stress taper, and (adjoint) kernel accumulation fused
+//                memory-variable update, stress taper, forward-field store (optionally at a
+//                lower polynomial degree) and (adjoint) kernel accumulation fused
 //   k_forces     point forces / adjoint sources added to the element force
 //   k_node       gather the element forces of the <=8 elements sharing a node in the
 //                reference's assembly order, divide by the mass, velocity update, taper
