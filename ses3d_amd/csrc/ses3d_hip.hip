@@ -1882,6 +1882,37 @@ __global__ void k_project(const float *__restrict__ u, int NX, int NY, long long
     out[t] = acc;
 }
 
+// Segment buffer of every event: the sampled steps of the longest segment of
+// k steps.  Table of one segment's sampled steps into it.
+int ensure_segment_buffer(ses3d_hip_context *f, int k) {
+    const int nt = f->cfg.nt, nseg = (nt + k - 1) / k;
+    int max_slots = 0;
+    const EventHost &E0 = f->evh[0];
+    for (int sgi = 0; sgi < nseg; ++sgi) {
+        int n = 0;
+        for (int it = sgi * k + 1; it <= std::min((sgi + 1) * k, nt); ++it) n += E0.slot_h[it] >= 0;
+        max_slots = std::max(max_slots, n);
+    }
+    for (auto &E : f->evh)
+        if (E.seg_slots < max_slots) {
+            if (E.seg) {
+                HIP_TRY(hipStreamSynchronize(f->stream));
+                dfree(f, E.seg);
+            }
+            TRY(dalloc(f, &E.seg, (size_t)max_slots * hist_slot_floats(f)));
+            E.seg_slots = max_slots;
+        }
+    return 0;
+}
+
+std::vector<int> segment_table(const EventHost &E, int nt, int s0, int s1) {
+    std::vector<int> tab(nt + 1, -1);
+    int n = 0;
+    for (int it = s0; it <= s1; ++it)
+        if (E.slot_h[it] >= 0) tab[it] = n++;
+    return tab;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -2503,37 +2534,6 @@ int ses3d_hip_get_saving_vector(ses3d_hip_context *c, int32_t *sv) {
     if (!c || !sv) return fail(SES3D_HIP_EINVAL, "null argument");
     for (int it = 1; it <= c->cfg.nt; ++it) sv[it - 1] = c->saving_vector.empty() ? 0 : c->saving_vector[it];
     return 0;
-}
-
-// Segment buffer of every event: the sampled steps of the longest segment of
-// k steps.  Table of one segment's sampled steps into it.
-int ensure_segment_buffer(ses3d_hip_context *f, int k) {
-    const int nt = f->cfg.nt, nseg = (nt + k - 1) / k;
-    int max_slots = 0;
-    const EventHost &E0 = f->evh[0];
-    for (int sgi = 0; sgi < nseg; ++sgi) {
-        int n = 0;
-        for (int it = sgi * k + 1; it <= std::min((sgi + 1) * k, nt); ++it) n += E0.slot_h[it] >= 0;
-        max_slots = std::max(max_slots, n);
-    }
-    for (auto &E : f->evh)
-        if (E.seg_slots < max_slots) {
-            if (E.seg) {
-                HIP_TRY(hipStreamSynchronize(f->stream));
-                dfree(f, E.seg);
-            }
-            TRY(dalloc(f, &E.seg, (size_t)max_slots * hist_slot_floats(f)));
-            E.seg_slots = max_slots;
-        }
-    return 0;
-}
-
-std::vector<int> segment_table(const EventHost &E, int nt, int s0, int s1) {
-    std::vector<int> tab(nt + 1, -1);
-    int n = 0;
-    for (int it = s0; it <= s1; ++it)
-        if (E.slot_h[it] >= 0) tab[it] = n++;
-    return tab;
 }
 
 int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
