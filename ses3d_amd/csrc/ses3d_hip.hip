@@ -58,11 +58,13 @@ int fail(int code, const std::string &msg) {
 #ifndef SES3D_EPB
 #define SES3D_EPB 1
 #endif
-// minimum waves per SIMD the element kernels are compiled for (5 fits every
-// variant in 96 VGPRs without spilling)
+// minimum waves per SIMD the element kernels are compiled for: 5 fits them in
+// 96 VGPRs without spilling, except the field-storing kernel with 2 elements
+// per workgroup, which is compiled for 4
 #ifndef SES3D_WAVES_PER_EU
 #define SES3D_WAVES_PER_EU 5
 #endif
+#define WAVES_MIN(MODE, EPB) ((MODE) == 1 && (EPB) == 2 ? 4 : (SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1))
 // Launch schedule: elements per element-kernel workgroup (1, 2 or 4; 128
 // lanes each, 125 active), lanes per node-kernel workgroup (128, 256 or 512:
 // 2, 4 or 8 node blocks of 64) and the XCD-aware workgroup order.  Every
@@ -173,11 +175,15 @@ __device__ __forceinline__ float ldnt(const float *p) {
 // 96x96x48 workload: element kernel -7 to -10%.
 typedef __attribute__((address_space(1))) const char cgchar;
 typedef __attribute__((address_space(1))) char gchar;
+template <int NT>
 __device__ __forceinline__ float ldo(const float *base, unsigned boff) {
-    return __builtin_nontemporal_load((cgfloat *)((cgchar *)base + boff));
+    if (NT) return __builtin_nontemporal_load((cgfloat *)((cgchar *)base + boff));
+    return *(cgfloat *)((cgchar *)base + boff);
 }
+template <int NT>
 __device__ __forceinline__ void sto(float *base, unsigned boff, float v) {
-    __builtin_nontemporal_store(v, (gfloat *)((gchar *)base + boff));
+    if (NT) __builtin_nontemporal_store(v, (gfloat *)((gchar *)base + boff));
+    else *(gfloat *)((gchar *)base + boff) = v;
 }
 
 // Paired components: two components of one point side by side, so a lane
@@ -193,14 +199,18 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) d2v gd2v;
 typedef __attribute__((address_space(1))) const f2v cgf2v;
 typedef __attribute__((address_space(1))) f2v gf2v;
+template <int NT>
 __device__ __forceinline__ f2v ldo2(const float *base, unsigned boff) {
-    return __builtin_nontemporal_load((cgf2v *)((cgchar *)base + boff));
+    if (NT) return __builtin_nontemporal_load((cgf2v *)((cgchar *)base + boff));
+    return *(cgf2v *)((cgchar *)base + boff);
 }
+template <int NT>
 __device__ __forceinline__ void sto2(float *base, unsigned boff, float a, float b) {
     f2v v;
     v.x = a;
     v.y = b;
-    __builtin_nontemporal_store(v, (gf2v *)((gchar *)base + boff));
+    if (NT) __builtin_nontemporal_store(v, (gf2v *)((gchar *)base + boff));
+    else *(gf2v *)((gchar *)base + boff) = v;
 }
 // the element force: written by the element kernel and read back by the node
 // kernel right after, so through the caches (plain stores and loads)
@@ -238,28 +248,32 @@ __host__ __device__ constexpr long long c9_offset(int c, long long NP) {
 }
 __host__ __device__ constexpr int c9_stride(int c) { return c == EZY ? 1 : 2; }
 // inner: the six components every element touches (planes 0..2)
+template <int NT>
 __device__ __forceinline__ void ld9_inner(const float *b, size_t NP, unsigned pb2, float *x) {
-    f2v t = ldo2(b, pb2);
+    f2v t = ldo2<NT>(b, pb2);
     x[EXX] = t.x; x[EYY] = t.y;
-    t = ldo2(b + 2 * NP, pb2);
+    t = ldo2<NT>(b + 2 * NP, pb2);
     x[EZZ] = t.x; x[EXY] = t.y;
-    t = ldo2(b + 4 * NP, pb2);
+    t = ldo2<NT>(b + 4 * NP, pb2);
     x[EXZ] = t.x; x[EYZ] = t.y;
 }
 // outer: the second members of the split pairs (plane 3 and EZY)
+template <int NT>
 __device__ __forceinline__ void ld9_outer(const float *b, size_t NP, unsigned pb2, unsigned pb, float *x) {
-    const f2v t = ldo2(b + 6 * NP, pb2);
+    const f2v t = ldo2<NT>(b + 6 * NP, pb2);
     x[EYX] = t.x; x[EZX] = t.y;
-    x[EZY] = ldo(b + 8 * NP, pb);
+    x[EZY] = ldo<NT>(b + 8 * NP, pb);
 }
+template <int NT>
 __device__ __forceinline__ void st9_inner(float *b, size_t NP, unsigned pb2, const float *x) {
-    sto2(b, pb2, x[EXX], x[EYY]);
-    sto2(b + 2 * NP, pb2, x[EZZ], x[EXY]);
-    sto2(b + 4 * NP, pb2, x[EXZ], x[EYZ]);
+    sto2<NT>(b, pb2, x[EXX], x[EYY]);
+    sto2<NT>(b + 2 * NP, pb2, x[EZZ], x[EXY]);
+    sto2<NT>(b + 4 * NP, pb2, x[EXZ], x[EYZ]);
 }
+template <int NT>
 __device__ __forceinline__ void st9_outer(float *b, size_t NP, unsigned pb2, unsigned pb, const float *x) {
-    sto2(b + 6 * NP, pb2, x[EYX], x[EZX]);
-    sto(b + 8 * NP, pb, x[EZY]);
+    sto2<NT>(b + 6 * NP, pb2, x[EYX], x[EZX]);
+    sto<NT>(b + 8 * NP, pb, x[EZY]);
 }
 
 __device__ __forceinline__ void stnt(float *p, float v) {
@@ -310,7 +324,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // One element-kernel workgroup: EPB elements of one event.  `blk` is the
 // workgroup's index among the section's blocks_per_event * n_events tiles.
 // MT / ET: MeshDev / EventDev, plain or in the constant address space.
-template <int DISS, int MODE, int ANISO, int EPB, typename MT, typename ET>
+template <int DISS, int MODE, int ANISO, int EPB, int NT, typename MT, typename ET>
 __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const int it, const int blk,
                                              const int blocks_per_event) {
     const int ev = blk / blocks_per_event;
@@ -362,29 +376,29 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
         vy = vxy.y;
         vz = ldnt(E.v + 2 * Ms.NGB + g);
         if (keep_d) {
-            ld9_inner(E.d, NP, pb2, d9);
-            ld9_outer(E.d, NP, pb2, pb, d9);
+            ld9_inner<NT>(E.d, NP, pb2, d9);
+            ld9_outer<NT>(E.d, NP, pb2, pb, d9);
         }
-        ld9_inner(E.s, NP, pb2, s9);
-        if (pml) ld9_outer(E.s, NP, pb2, pb, s9);
+        ld9_inner<NT>(E.s, NP, pb2, s9);
+        if (pml) ld9_outer<NT>(E.s, NP, pb2, pb, s9);
         if (DISS) {
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
-                const f2v t = ldo2(E.M + 2 * q * NP, pb2);
+                const f2v t = ldo2<NT>(E.M + 2 * q * NP, pb2);
                 Mv[2 * q] = t.x;
                 Mv[2 * q + 1] = t.y;
             }
-            QQ = ldo(Ms.Q, pb);
+            QQ = ldo<NT>(Ms.Q, pb);
         }
         {
-            const f2v t = ldo2(Ms.lm, pb2);
+            const f2v t = ldo2<NT>(Ms.lm, pb2);
             lam = t.x;
             mu = t.y;
         }
         if (ANISO) {
-            A = ldo(Ms.A, pb);
-            B = ldo(Ms.B, pb);
-            C = ldo(Ms.C, pb);
+            A = ldo<NT>(Ms.A, pb);
+            B = ldo<NT>(Ms.B, pb);
+            C = ldo<NT>(Ms.C, pb);
         }
         if (MODE == 2 && hcomp) {
             // the element's stored values of the paired forward step, [field][node]
@@ -447,11 +461,11 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             dxuy = d9[EXY] + dt * exy;
             dyux = d9[EYX] + dt * eyx;
             float *d = E.d;
-            sto2(d, pb2, dxux, dyuy);
-            sto2(d + 2 * NP, pb2, dzuz, dxuy);
-            sto2(d + 4 * NP, pb2, dxuz, dyuz);
-            sto2(d + 6 * NP, pb2, dyux, dzux);
-            sto(d + 8 * NP, pb, dzuy);
+            sto2<NT>(d, pb2, dxux, dyuy);
+            sto2<NT>(d + 2 * NP, pb2, dzuz, dxuy);
+            sto2<NT>(d + 4 * NP, pb2, dxuz, dyuz);
+            sto2<NT>(d + 6 * NP, pb2, dyux, dzux);
+            sto<NT>(d + 8 * NP, pb, dzuy);
         }
         // absorbing-layer terms (:123-134); exact no-ops outside the layers
         float px = 0.f, py = 0.f, pz = 0.f, taper = 1.f;
@@ -516,7 +530,7 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
                     const float q = Ms.fast_tau ? div_recip(dM, Ms.tau_p[k], Ms.rtau_p[k]) : dM / Ms.tau_p[k];
                     const float Mn = (Mo - q) - Ms.Lk[k] * ev6[c];
                     Mv[c * 3 + k] = Mn;
-                    if ((c * 3 + k) & 1) sto2(E.M + (c * 3 + k - 1) * NP, pb2, Mv[c * 3 + k - 1], Mn);
+                    if ((c * 3 + k) & 1) sto2<NT>(E.M + (c * 3 + k - 1) * NP, pb2, Mv[c * 3 + k - 1], Mn);
                 }
         } else {
             const float L = lam * ((exx + eyy) + ezz);
@@ -578,10 +592,10 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
         if (pml) {
 #pragma unroll
             for (int c = 0; c < 9; ++c) s9[c] = s9[c] * taper;
-            st9_inner(s, NP, pb2, s9);
-            st9_outer(s, NP, pb2, pb, s9);
+            st9_inner<NT>(s, NP, pb2, s9);
+            st9_outer<NT>(s, NP, pb2, pb, s9);
         } else {
-            st9_inner(s, NP, pb2, s9);
+            st9_inner<NT>(s, NP, pb2, s9);
         }
 
         if (MODE == 1 && hcomp) {
@@ -599,11 +613,11 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             const int sl = ldg(E.slot + it);
             if (sl >= 0) {
                 float *h = E.hist + (size_t)sl * 9 * NP;
-                sto2(h, pb2, vx, vy);
-                sto2(h + 2 * NP, pb2, vz, dxux);
-                sto2(h + 4 * NP, pb2, dyuy, dzuz);
-                sto2(h + 6 * NP, pb2, (dxuy + dyux) / 2.0f, (dxuz + dzux) / 2.0f);
-                sto(h + 8 * NP, pb, (dyuz + dzuy) / 2.0f);
+                sto2<NT>(h, pb2, vx, vy);
+                sto2<NT>(h + 2 * NP, pb2, vz, dxux);
+                sto2<NT>(h + 4 * NP, pb2, dyuy, dzuz);
+                sto2<NT>(h + 6 * NP, pb2, (dxuy + dyux) / 2.0f, (dxuz + dzux) / 2.0f);
+                sto<NT>(h + 8 * NP, pb, (dyuz + dzuy) / 2.0f);
             }
         }
         if (MODE == 2) {
@@ -635,11 +649,11 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
                     const float *h = E.hist + (size_t)sl * 9 * NP;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const f2v t = ldo2(h + 2 * q * NP, pb2);
+                        const f2v t = ldo2<NT>(h + 2 * q * NP, pb2);
                         f9[2 * q] = t.x;
                         f9[2 * q + 1] = t.y;
                     }
-                    f9[8] = ldo(h + 8 * NP, pb);
+                    f9[8] = ldo<NT>(h + 8 * NP, pb);
                 }
                 const float vxf = f9[0], vyf = f9[1], vzf = f9[2];
                 const float exxf = f9[3], eyyf = f9[4], ezzf = f9[5];
@@ -741,13 +755,14 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
 }
 
 // One section: the step number lives in HBM so a captured step graph replays unchanged.
-template <int DISS, int MODE, int ANISO, int EPB>
+// NT: the state streams bypass the caches (sections larger than the caches).
+template <int DISS, int MODE, int ANISO, int EPB, int NT>
 __global__ __launch_bounds__(EPB * TPE)
-__attribute__((amdgpu_waves_per_eu(SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(WAVES_MIN(MODE, EPB), 8)))
 void k_element(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs, const int *itp,
                int blocks_per_event, int remap) {
     const int blk = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    element_step<DISS, MODE, ANISO, EPB>(*mesh, evs, *itp, blk, blocks_per_event);
+    element_step<DISS, MODE, ANISO, EPB, NT>(*mesh, evs, *itp, blk, blocks_per_event);
 }
 
 // Several sections (meshes of any size, each with its own events and step
@@ -776,14 +791,14 @@ __device__ __forceinline__ int section_of(cSectionDev *secs, int nsec, int b, bo
     return s;
 }
 
-template <int DISS, int MODE, int ANISO, int EPB>
+template <int DISS, int MODE, int ANISO, int EPB, int NT>
 __global__ __launch_bounds__(EPB * TPE)
-__attribute__((amdgpu_waves_per_eu(SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(WAVES_MIN(MODE, EPB), 8)))
 void k_element_group(const SectionDev *__restrict__ secs, int nsec, int remap) {
     const int b = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     cSectionDev *cs = (cSectionDev *)secs;
     cSectionDev &S = cs[section_of(cs, nsec, b, false)];
-    element_step<DISS, MODE, ANISO, EPB>(*(cMeshDev *)S.mesh, (cEventDev *)S.evs, *S.itp, b - S.eb0, S.bpe);
+    element_step<DISS, MODE, ANISO, EPB, NT>(*(cMeshDev *)S.mesh, (cEventDev *)S.evs, *S.itp, b - S.eb0, S.bpe);
 }
 
 // ---------------------------------------------------------------------------
@@ -1365,6 +1380,15 @@ int zero(ses3d_hip_context *c, void *p, size_t bytes) {
 
 int elem_blocks(const ses3d_hip_context *c, int epb) { return (int)((c->NE + epb - 1) / epb); }
 long long node_blocks(const ses3d_hip_context *c) { return (long long)c->NBX * c->NBY * c->NBZ; }
+// Bytes of element-local state one step streams (all events): displacement
+// gradient, split stresses, memory variables, element force, coefficients.
+// Above the MI355X's 256 MiB last-level cache the element kernel reads and
+// writes them non-temporally; below it they stay cached between steps.
+constexpr long long CACHED_STATE_BYTES = 256LL << 20;
+long long stream_bytes(const ses3d_hip_context *c) {
+    const long long per_event = 9 + 9 + (c->cfg.is_diss ? 18 : 0) + 3;
+    return 4LL * c->NP * (per_event * c->n_events + 4);
+}
 // floats of one sampled step of stored forward fields: 9 fields of NP values,
 // or of NE*(hdeg+1)^3 values when stored at a lower degree
 long long hist_slot_floats(const ses3d_hip_context *c) {
@@ -1383,16 +1407,23 @@ struct ElemLaunch {
     int bpe;
     const SectionDev *secs;
     int nsec;
+    bool nt;    // state streams non-temporal
 };
+
+template <int D, int MO, int AN, int EP, int NT>
+void launch_elem_nt(const ElemLaunch &L) {
+    if (L.secs)
+        hipLaunchKernelGGL((k_element_group<D, MO, AN, EP, NT>), dim3(L.wgs), dim3(EP * TPE), 0, L.stream, L.secs,
+                           L.nsec, L.remap);
+    else
+        hipLaunchKernelGGL((k_element<D, MO, AN, EP, NT>), dim3(L.wgs), dim3(EP * TPE), 0, L.stream, L.mesh, L.evs,
+                           L.itp, L.bpe, L.remap);
+}
 
 template <int D, int MO, int AN, int EP>
 void launch_elem_epb(const ElemLaunch &L) {
-    if (L.secs)
-        hipLaunchKernelGGL((k_element_group<D, MO, AN, EP>), dim3(L.wgs), dim3(EP * TPE), 0, L.stream, L.secs, L.nsec,
-                           L.remap);
-    else
-        hipLaunchKernelGGL((k_element<D, MO, AN, EP>), dim3(L.wgs), dim3(EP * TPE), 0, L.stream, L.mesh, L.evs, L.itp,
-                           L.bpe, L.remap);
+    if (L.nt) launch_elem_nt<D, MO, AN, EP, 1>(L);
+    else launch_elem_nt<D, MO, AN, EP, 0>(L);
 }
 
 template <int D, int MO, int AN>
@@ -1594,7 +1625,8 @@ int ses3d_hip_context::set_step_on(hipStream_t st, int it) {
 
 int ses3d_hip_context::launch_element(int mode) {
     const int bpe = elem_blocks(this, sched.epb);
-    ElemLaunch L{stream, sched.epb, sched.remap_elem, (unsigned)(bpe * n_events), mesh_d, ev_d, it_d, bpe, nullptr, 0};
+    ElemLaunch L{stream, sched.epb, sched.remap_elem, (unsigned)(bpe * n_events), mesh_d, ev_d, it_d, bpe, nullptr, 0,
+                 stream_bytes(this) > CACHED_STATE_BYTES};
     return launch_elem(cfg.is_diss, mode, aniso ? 1 : 0, L);
 }
 
@@ -1634,8 +1666,10 @@ int ses3d_hip_group::build_table() {
 }
 
 int ses3d_hip_group::launch_element(int mode) {
+    long long bytes = 0;
+    for (auto *c : ctx) bytes += stream_bytes(c);
     ElemLaunch L{stream, sched.epb, sched.remap_elem, (unsigned)elem_wgs, nullptr, nullptr, nullptr, 0, secs_d,
-                 (int)ctx.size()};
+                 (int)ctx.size(), bytes > CACHED_STATE_BYTES};
     return launch_elem(diss, mode, aniso, L);
 }
 
