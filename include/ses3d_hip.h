@@ -219,6 +219,17 @@ int ses3d_hip_get_timing(ses3d_hip_context *ctx, double *total_ms, double *eleme
  * node kernel times (12 floats, -1 before any autotune). */
 int ses3d_hip_set_schedule(ses3d_hip_context *ctx, const int32_t *sched);
 int ses3d_hip_get_schedule(ses3d_hip_context *ctx, int32_t *sched, float *trial_ms);
+
+/* Cache policy of the element kernel's state streams (no counterpart in the
+ * reference): mode -1 (default) streams the displacement gradient, split
+ * stresses, memory variables, coefficients and sampled fields with
+ * non-temporal loads and stores when they exceed the 256 MiB last-level cache
+ * and keeps them cached otherwise; 0 always cached; 1 always non-temporal.
+ * Both variants give bit-identical results.  get_streaming returns the mode
+ * and, in *in_use, whether the non-temporal variant runs (either pointer may
+ * be NULL). */
+int ses3d_hip_set_streaming(ses3d_hip_context *ctx, int32_t mode);
+int ses3d_hip_get_streaming(ses3d_hip_context *ctx, int32_t *mode, int32_t *in_use);
 int ses3d_hip_autotune(ses3d_hip_context *ctx, int32_t it_begin, int32_t n_steps);
 
 /* Groups: several initialised contexts (sections of any size, each with its
@@ -240,6 +251,8 @@ int ses3d_hip_group_autotune(ses3d_hip_group *grp, int32_t it_begin, int32_t n_s
 int ses3d_hip_group_synchronize(ses3d_hip_group *grp);
 int ses3d_hip_group_set_schedule(ses3d_hip_group *grp, const int32_t *sched);
 int ses3d_hip_group_get_schedule(ses3d_hip_group *grp, int32_t *sched, float *trial_ms);
+int ses3d_hip_group_set_streaming(ses3d_hip_group *grp, int32_t mode);   /* by the members' summed state */
+int ses3d_hip_group_get_streaming(ses3d_hip_group *grp, int32_t *mode, int32_t *in_use);
 int ses3d_hip_group_set_graph_steps(ses3d_hip_group *grp, int32_t steps);
 int ses3d_hip_group_set_profiling(ses3d_hip_group *grp, int32_t on);
 int ses3d_hip_group_get_timing(ses3d_hip_group *grp, double *total_ms, double *element_ms,

@@ -1124,6 +1124,7 @@ struct StepRunner {
     Schedule sched;
     int it_host = -1;                 // value the device step counter(s) hold next (-1 unknown)
     int graph_steps = 16;
+    int streaming = -1;               // element-kernel state streams: -1 by state size, 0 cached, 1 non-temporal
     std::vector<GraphEntry> graphs;
     bool profiling = false;
     double t_total = 0, t_elem = 0, t_node = 0;
@@ -1389,6 +1390,15 @@ long long stream_bytes(const ses3d_hip_context *c) {
     const long long per_event = 9 + 9 + (c->cfg.is_diss ? 18 : 0) + 3;
     return 4LL * c->NP * (per_event * c->n_events + 4);
 }
+bool use_streaming(const StepRunner *r, long long bytes) {
+    return r->streaming < 0 ? bytes > CACHED_STATE_BYTES : r->streaming != 0;
+}
+bool use_streaming(const ses3d_hip_context *c) { return use_streaming(c, stream_bytes(c)); }
+bool use_streaming(const ses3d_hip_group *g) {
+    long long bytes = 0;
+    for (auto *c : g->ctx) bytes += stream_bytes(c);
+    return use_streaming(g, bytes);
+}
 // floats of one sampled step of stored forward fields: 9 fields of NP values,
 // or of NE*(hdeg+1)^3 values when stored at a lower degree
 long long hist_slot_floats(const ses3d_hip_context *c) {
@@ -1626,7 +1636,7 @@ int ses3d_hip_context::set_step_on(hipStream_t st, int it) {
 int ses3d_hip_context::launch_element(int mode) {
     const int bpe = elem_blocks(this, sched.epb);
     ElemLaunch L{stream, sched.epb, sched.remap_elem, (unsigned)(bpe * n_events), mesh_d, ev_d, it_d, bpe, nullptr, 0,
-                 stream_bytes(this) > CACHED_STATE_BYTES};
+                 use_streaming(this)};
     return launch_elem(cfg.is_diss, mode, aniso ? 1 : 0, L);
 }
 
@@ -1666,10 +1676,8 @@ int ses3d_hip_group::build_table() {
 }
 
 int ses3d_hip_group::launch_element(int mode) {
-    long long bytes = 0;
-    for (auto *c : ctx) bytes += stream_bytes(c);
     ElemLaunch L{stream, sched.epb, sched.remap_elem, (unsigned)elem_wgs, nullptr, nullptr, nullptr, 0, secs_d,
-                 (int)ctx.size(), bytes > CACHED_STATE_BYTES};
+                 (int)ctx.size(), use_streaming(this)};
     return launch_elem(diss, mode, aniso, L);
 }
 
@@ -1714,6 +1722,15 @@ int set_schedule(StepRunner *r, const int32_t *v) {
     HIP_TRY(hipStreamSynchronize(r->stream));
     r->sched = s;
     return r->schedule_changed();
+}
+
+int set_streaming(StepRunner *r, int32_t mode) {
+    if (mode < -1 || mode > 1)
+        return fail(SES3D_HIP_EINVAL, "streaming: -1 (by state size), 0 (cached) or 1 (non-temporal)");
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    r->streaming = mode;
+    r->drop_graphs();
+    return 0;
 }
 
 void get_schedule(const StepRunner *r, int32_t *v, float *trial_ms) {
@@ -2364,6 +2381,18 @@ int ses3d_hip_get_schedule(ses3d_hip_context *c, int32_t *sched, float *trial_ms
     return 0;
 }
 
+int ses3d_hip_set_streaming(ses3d_hip_context *c, int32_t mode) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    return set_streaming(c, mode);
+}
+
+int ses3d_hip_get_streaming(ses3d_hip_context *c, int32_t *mode, int32_t *in_use) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (mode) *mode = c->streaming;
+    if (in_use) *in_use = use_streaming(c) ? 1 : 0;
+    return 0;
+}
+
 int ses3d_hip_synchronize(ses3d_hip_context *c) {
     if (!c) return fail(SES3D_HIP_EINVAL, "null context");
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -2834,6 +2863,18 @@ int ses3d_hip_group_set_schedule(ses3d_hip_group *g, const int32_t *sched) {
 int ses3d_hip_group_get_schedule(ses3d_hip_group *g, int32_t *sched, float *trial_ms) {
     if (!g || !sched) return fail(SES3D_HIP_EINVAL, "null argument");
     get_schedule(g, sched, trial_ms);
+    return 0;
+}
+
+int ses3d_hip_group_set_streaming(ses3d_hip_group *g, int32_t mode) {
+    if (!g) return fail(SES3D_HIP_EINVAL, "null group");
+    return set_streaming(g, mode);
+}
+
+int ses3d_hip_group_get_streaming(ses3d_hip_group *g, int32_t *mode, int32_t *in_use) {
+    if (!g) return fail(SES3D_HIP_EINVAL, "null group");
+    if (mode) *mode = g->streaming;
+    if (in_use) *in_use = use_streaming(g) ? 1 : 0;
     return 0;
 }
 

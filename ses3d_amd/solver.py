@@ -87,6 +87,8 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_set_schedule": ([P, ip], C.c_int),
         "ses3d_hip_get_schedule": ([P, ip, fp], C.c_int),
         "ses3d_hip_autotune": ([P, i32, i32], C.c_int),
+        "ses3d_hip_set_streaming": ([P, i32], C.c_int),
+        "ses3d_hip_get_streaming": ([P, ip, ip], C.c_int),
         "ses3d_hip_group_create": ([C.POINTER(P), i32, C.POINTER(P)], C.c_int),
         "ses3d_hip_group_destroy": ([P], None),
         "ses3d_hip_group_run": ([P, i32, i32], C.c_int),
@@ -94,6 +96,8 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_group_synchronize": ([P], C.c_int),
         "ses3d_hip_group_set_schedule": ([P, ip], C.c_int),
         "ses3d_hip_group_get_schedule": ([P, ip, fp], C.c_int),
+        "ses3d_hip_group_set_streaming": ([P, i32], C.c_int),
+        "ses3d_hip_group_get_streaming": ([P, ip, ip], C.c_int),
         "ses3d_hip_group_set_graph_steps": ([P, i32], C.c_int),
         "ses3d_hip_group_set_profiling": ([P, i32], C.c_int),
         "ses3d_hip_group_get_timing": ([P, dp, dp, dp, ip], C.c_int),
@@ -232,6 +236,14 @@ class Solver:
 
     def schedule(self) -> dict:
         return _schedule(self.lib.ses3d_hip_get_schedule, self.h)
+
+    def set_streaming(self, mode: int = -1):
+        """Element-kernel state streams: -1 non-temporal when they exceed the
+        last-level cache (default), 0 always cached, 1 always non-temporal."""
+        _check(self.lib.ses3d_hip_set_streaming(self.h, int(mode)))
+
+    def streaming(self) -> dict:
+        return _streaming(self.lib.ses3d_hip_get_streaming, self.h)
 
     def synchronize(self):
         _check(self.lib.ses3d_hip_synchronize(self.h))
@@ -376,6 +388,12 @@ def _schedule(fn, h) -> dict:
             "trial_ms": ms.reshape(6, 2).tolist() if ms[0] >= 0 else None}
 
 
+def _streaming(fn, h) -> dict:
+    mode, in_use = C.c_int32(), C.c_int32()
+    _check(fn(h, C.byref(mode), C.byref(in_use)))
+    return {"mode": mode.value, "non_temporal": bool(in_use.value)}
+
+
 class Group:
     """Sections (initialised Solvers, each with its own mesh and events)
     advanced together: one element-kernel and one node-kernel launch per step
@@ -430,6 +448,12 @@ class Group:
 
     def schedule(self) -> dict:
         return _schedule(self.lib.ses3d_hip_group_get_schedule, self.h)
+
+    def set_streaming(self, mode: int = -1):
+        _check(self.lib.ses3d_hip_group_set_streaming(self.h, int(mode)))
+
+    def streaming(self) -> dict:
+        return _streaming(self.lib.ses3d_hip_group_get_streaming, self.h)
 
     def set_graph_steps(self, steps: int):
         _check(self.lib.ses3d_hip_group_set_graph_steps(self.h, int(steps)))

@@ -70,4 +70,9 @@ def test_group_and_schedule_entry_points_reject_null():
     sched = (ctypes.c_int32 * 4)(2, 256, 1, 1)
     assert lib.ses3d_hip_set_schedule(None, sched) != 0
     assert lib.ses3d_hip_autotune(None, 1, 6) != 0
+    assert lib.ses3d_hip_set_streaming(None, 1) != 0
+    assert lib.ses3d_hip_get_streaming(None, None, None) != 0
+    assert lib.ses3d_hip_group_set_streaming(None, 0) != 0
+    assert lib.ses3d_hip_group_get_streaming(None, None, None) != 0
+    assert b"null" in lib.ses3d_hip_last_error()
     lib.ses3d_hip_group_destroy(None)
