@@ -69,6 +69,27 @@ def implemented_bytes(nel, pml: int, is_diss: bool, aniso: bool = False) -> floa
     return inner * (common + 4 * 12) + (1 - inner) * (common + 4 * 36)
 
 
+def device_copy_gbps(mib: int = 2048, reps: int = 10) -> float:
+    """HBM bandwidth a plain device-to-device copy reaches on this GPU (bytes
+    read + written per second, GB/s): the practical ceiling a streaming
+    kernel is compared with, next to the 8 TB/s data-sheet peak."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    t1.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * 4 * n / (ms * 1e-3) / 1e9
+
+
 def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
     """Time the reference binary on its largest one-rank mesh (its static
     arrays hold 23 x 28 x 8 elements, SOURCE/ses3d_modules.f90:13-15);
@@ -419,6 +440,12 @@ def main():
         "schedule": sched,
     }
     s.close()
+    copy = device_copy_gbps()
+    roof = out["roofline"]
+    roof["device_copy_GBps"] = copy
+    if traffic:
+        roof["traffic_GBps"] = traffic / (elem_ms * 1e-3) / 1e9
+        roof["traffic_frac_of_copy"] = roof["traffic_GBps"] / copy
     if not args.no_elastic:
         print("# elastic 24x24x12 leg", file=sys.stderr, flush=True)
         out["elastic_24x24x12"] = small_forward_leg()
