@@ -129,7 +129,7 @@ def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
                        f"({times[1]:.1f} s and {times[0]:.1f} s)")}
 
 
-def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int = 4) -> dict:
+def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int = 4, kernel_bits: int = 64) -> dict:
     """Seconds per event-gradient on the differentiation workload (48x48x24
     anelastic, dt=0.15): data from the 'true' model (1D + 2% Gaussian blobs),
     gradient of the L2 waveform misfit at the 1D model by the checkpointed
@@ -138,7 +138,9 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int =
     100 surface receivers; the others seeded sources 5-600 km deep with 10-400
     receivers).  The timed region is forward + adjoint of all events.
     history_degree < 4 stores the sampled forward fields at that polynomial
-    degree (ses3d_store's `deg`; lossy, smaller segment buffer)."""
+    degree (ses3d_store's `deg`; lossy, smaller segment buffer); kernel_bits=32
+    accumulates the kernels in float32 as the reference does (its kernels bit
+    for bit) instead of float64."""
     from dataclasses import replace
     from ses3d_amd import configs, misfit
     from ses3d_amd.solver import Solver
@@ -163,6 +165,7 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int =
     f.init()
     a = Solver.from_run(cur, adjoint_flag=2, pml=w.pml, nt=nt, n_events=n_events)
     a.set_history_degree(history_degree)
+    a.set_kernel_precision(kernel_bits)
     a.set_model(cur.model)
     dt = float(cur.events[0].dt)
     taper = misfit.cosine_taper(nt, dt, 40.0)
@@ -186,6 +189,7 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int =
            "device_GiB": (f.memory_bytes() + a.memory_bytes()) / 2 ** 30}
     if history_degree < 4:
         out["history_degree"] = history_degree
+    out["kernel_accumulators"] = f"float{kernel_bits}"
     f.close()
     a.close()
     return out
@@ -456,6 +460,10 @@ def main():
         gb = gradient_leg(k=args.checkpoint_every, n_events=args.gradient_events, history_degree=args.history_degree)
         gb["per_event_speedup_over_plain"] = out["gradient"]["s_per_event_gradient"] / gb["s_per_event_gradient"]
         out["gradient_batched"] = gb
+        print("# gradient leg, float32 kernel accumulators", file=sys.stderr, flush=True)
+        g32 = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree, kernel_bits=32)
+        g32["speedup_over_float64"] = out["gradient"]["s_per_event_gradient"] / g32["s_per_event_gradient"]
+        out["gradient_f32"] = g32
     if not args.no_batched:
         print("# batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg()

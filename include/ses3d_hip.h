@@ -137,10 +137,13 @@ int ses3d_hip_get_seismograms(ses3d_hip_context *ctx, int32_t ev, float *out);
  */
 int ses3d_hip_get_field(ses3d_hip_context *ctx, int32_t ev, const char *name, float *out);
 
-/* Frechet kernels of event ev accumulated in float64, reference layout:
- * grad_rho, grad_cp, grad_csh, grad_csv (SOURCE/ses3d_grad.f90:77-83) and,
- * when is_diss==1, grad_Q_mu (SOURCE/ses3d_grad_Q.f90:111-128,151-153); any
- * pointer may be NULL.  Replaces the kernel files of SOURCE/ses3d_output.f90:318-343. */
+/* Frechet kernels of event ev, reference layout: grad_rho, grad_cp,
+ * grad_csh, grad_csv (SOURCE/ses3d_grad.f90:77-83) and, when is_diss==1,
+ * grad_Q_mu (SOURCE/ses3d_grad_Q.f90:111-128,151-153); any pointer may be
+ * NULL.  Accumulated in float64 by default, or in float32 (the type of the
+ * reference's grad_* arrays, widened exactly on output) after
+ * ses3d_hip_set_kernel_precision.  Replaces the kernel files of
+ * SOURCE/ses3d_output.f90:318-343. */
 int ses3d_hip_get_kernels(ses3d_hip_context *ctx, int32_t ev, double *grad_rho, double *grad_cp,
                           double *grad_csh, double *grad_csv, double *grad_Q_mu);
 
@@ -169,6 +172,15 @@ int ses3d_hip_get_saving_vector(ses3d_hip_context *ctx, int32_t *sv);
  * the stored steps.  history_size gives the floats of one stored step. */
 int ses3d_hip_set_history_degree(ses3d_hip_context *ctx, int32_t deg);
 int ses3d_hip_history_size(ses3d_hip_context *ctx, int64_t *n);
+
+/* Type of the kernel accumulators of an adjoint_flag==2 context: bits = 64
+ * (default, float64) or 32 (float32, as the reference's grad_* arrays,
+ * SOURCE/ses3d_modules.f90:221-222, summed in the order of
+ * SOURCE/ses3d_grad.f90:77-83 and SOURCE/ses3d_grad_Q.f90:111-128, so the
+ * kernels equal the reference's bit for bit; 40 fewer bytes per point per
+ * sampled adjoint step).  Takes effect at the next init, which zeroes the
+ * kernels. */
+int ses3d_hip_set_kernel_precision(ses3d_hip_context *ctx, int32_t bits);
 
 /* Gradient without disk history: for an adjoint_flag==2 context, take the
  * forward wave field from HBM checkpoints.  ses3d_hip_forward_checkpointed

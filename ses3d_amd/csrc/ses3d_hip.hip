@@ -118,6 +118,8 @@ struct MeshDev {
     // values per element and field, hslot floats per sampled step
     int hdeg, hnc;
     long long hslot;
+    // kernel accumulators in float32 (the reference's type) instead of float64
+    int kf32;
 };
 
 struct EventDev {
@@ -133,7 +135,8 @@ struct EventDev {
     float *hist;           // sampled forward fields, 9*NP per slot: pair planes (vx,vy)
                            // (vz,exx) (eyy,ezz) (exy,exz), eyz at 8*NP (or degree < 4 values)
     const int *slot;       // [it] -> slot or -1 (nt+1 entries)
-    double *grad;          // 5*NP  pair planes (rho,cp) (csh,csv), Q_mu at 4*NP
+    double *grad;          // 5*NP  pair planes (rho,cp) (csh,csv), Q_mu at 4*NP; with
+                           // float32 accumulators the same layout in floats
     float sdt;             // samp_ad*dt
 };
 
@@ -674,7 +677,30 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
                 const float g_csh = (sdt * (((4.0f * rho) * cs) *
                                             (((ede - tra * trf) - (2.0f * exzf) * exza) - (2.0f * eyzf) * eyza))) / Jac;
                 const float g_csv = (sdt * (((8.0f * rho) * cs) * (exzf * exza + eyzf * eyza))) / Jac;
-                {
+                float gqi[3] = {0.f, 0.f, 0.f};
+                if (DISS) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float Mxx = Mv[0 * 3 + k], Myy = Mv[1 * 3 + k], Mzz = Mv[2 * 3 + k];
+                        const float Mxy = Mv[3 * 3 + k], Mxz = Mv[4 * 3 + k], Myz = Mv[5 * 3 + k];
+                        const float md = (((((Mxx * exxf + Myy * eyyf) + Mzz * ezzf) + Mxy * exyf) + Mxz * exzf) +
+                                          Myz * eyzf) - (trf * ((Mxx + Myy) + Mzz)) / 3.0f;
+                        gqi[k] = ((sdt * md) * Ms.tau_p[k]) / Jac;
+                    }
+                }
+                if (Ms.kf32) {
+                    // float32 sums in the reference's statement order (bit for bit its kernels)
+                    float *gf = (float *)gr;
+                    gf2v *g01 = (gf2v *)(gf + 2 * (size_t)p), *g23 = (gf2v *)(gf + 2 * NP + 2 * (size_t)p);
+                    f2v a = *g01, b = *g23;
+                    a.y = a.y + g_cp;
+                    a.x = (a.x + g_rho1) + g_rho2;
+                    b.x = b.x + g_csh;
+                    b.y = b.y + g_csv;
+                    *g01 = a;
+                    *g23 = b;
+                    if (DISS) gf[4 * NP + p] = ((gf[4 * NP + p] + gqi[0]) + gqi[1]) + gqi[2];
+                } else {
                     gd2v *g01 = (gd2v *)(gr + 2 * (size_t)p), *g23 = (gd2v *)(gr + 2 * NP + 2 * (size_t)p);
                     d2v a = *g01, b = *g23;
                     a.x = (a.x + (double)g_rho1) + (double)g_rho2;
@@ -683,18 +709,7 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
                     b.y = b.y + (double)g_csv;
                     *g01 = a;
                     *g23 = b;
-                }
-                if (DISS) {
-                    double gq = gr[4 * NP + p];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const float Mxx = Mv[0 * 3 + k], Myy = Mv[1 * 3 + k], Mzz = Mv[2 * 3 + k];
-                        const float Mxy = Mv[3 * 3 + k], Mxz = Mv[4 * 3 + k], Myz = Mv[5 * 3 + k];
-                        const float md = (((((Mxx * exxf + Myy * eyyf) + Mzz * ezzf) + Mxy * exyf) + Mxz * exzf) +
-                                          Myz * eyzf) - (trf * ((Mxx + Myy) + Mzz)) / 3.0f;
-                        gq = gq + (double)(((sdt * md) * Ms.tau_p[k]) / Jac);
-                    }
-                    gr[4 * NP + p] = gq;
+                    if (DISS) gr[4 * NP + p] = ((gr[4 * NP + p] + (double)gqi[0]) + (double)gqi[1]) + (double)gqi[2];
                 }
             }
         }
@@ -1196,6 +1211,7 @@ struct ses3d_hip_context : StepRunner {
     ses3d::MeshSetup ms;
     bool model_set = false, initialised = false;
     int hdeg = 4;                        // degree of the stored forward fields (4: unchanged)
+    int kernel_bits = 64;                // kernel accumulators: 64 float64, 32 float32 (the reference's)
     // device mesh data
     float *lm = nullptr, *A = nullptr, *B = nullptr, *C = nullptr, *Q = nullptr,
           *rhoinv = nullptr, *MM = nullptr;
@@ -2193,6 +2209,7 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     M.hdeg = c->hdeg;
     M.hnc = (c->hdeg + 1) * (c->hdeg + 1) * (c->hdeg + 1);
     M.hslot = hist_slot_floats(c);
+    M.kf32 = c->kernel_bits == 32;
     M.lm = c->lm; M.A = c->A; M.B = c->B; M.C = c->C; M.Q = c->Q; M.rhoinv = c->rhoinv;
     M.MM = c->MM; M.Z = c->Z; M.S = c->S; M.COT = c->COT; M.IX = c->IX; M.IY = c->IY; M.IZ = c->IZ;
     M.LUT = c->LUT;
@@ -2518,20 +2535,33 @@ int ses3d_hip_get_kernels(ses3d_hip_context *c, int32_t ev, double *g_rho, doubl
     EventHost &E = c->evh[ev];
     if (!E.grad) return fail(SES3D_HIP_ESTATE, "kernels exist only for adjoint_flag=2 contexts");
     const long long NP = c->NP;
+    const bool f32 = c->kernel_bits == 32;
     double *outs[5] = {g_rho, g_cp, g_csh, g_csv, g_Q};
+    std::vector<float> tmp(f32 ? c->NPTS : 0);
     for (int k = 0; k < 5; ++k) {
         if (!outs[k]) continue;
         if (k == 4 && !c->cfg.is_diss) return fail(SES3D_HIP_EINVAL, "grad_Q_mu needs is_diss=1");
         // pair planes (rho, cp) (csh, csv), then Q_mu
-        if (k < 4) TRY(download_ref(c, outs[k], E.grad + (k / 2) * 2 * NP + (k % 2), 2));
-        else TRY(download_ref(c, outs[k], E.grad + 4 * NP, 1));
+        const long long off = k < 4 ? (k / 2) * 2 * NP + (k % 2) : 4 * NP;
+        const int stride = k < 4 ? 2 : 1;
+        if (f32) {
+            TRY(download_ref(c, tmp.data(), (const float *)E.grad + off, stride));
+            for (long long i = 0; i < c->NPTS; ++i) outs[k][i] = (double)tmp[i];
+        } else {
+            TRY(download_ref(c, outs[k], E.grad + off, stride));
+        }
     }
     if (g_Q && c->cfg.is_diss) {
-        // normalisation at it==nt (SOURCE/ses3d_grad_Q.f90:151-153): 2 mu K / Q
+        // normalisation at it==nt (SOURCE/ses3d_grad_Q.f90:151-153): 2 mu K / Q,
+        // in the accumulators' type
         std::vector<float> mur(c->NPTS), qr(c->NPTS);
         TRY(download_ref(c, mur.data(), (const float *)c->lm + 1, 2));
         TRY(download_ref(c, qr.data(), (const float *)c->Q, 1));
-        for (long long i = 0; i < c->NPTS; ++i) g_Q[i] = ((2.0 * (double)mur[i]) * g_Q[i]) / (double)qr[i];
+        if (f32)
+            for (long long i = 0; i < c->NPTS; ++i)
+                g_Q[i] = (double)(((2.0f * mur[i]) * (float)g_Q[i]) / qr[i]);
+        else
+            for (long long i = 0; i < c->NPTS; ++i) g_Q[i] = ((2.0 * (double)mur[i]) * g_Q[i]) / (double)qr[i];
     }
     return 0;
 }
@@ -2602,6 +2632,16 @@ int ses3d_hip_set_history_degree(ses3d_hip_context *c, int32_t deg) {
     }
     c->hdeg = deg;
     c->initialised = false;
+    return 0;
+}
+
+int ses3d_hip_set_kernel_precision(ses3d_hip_context *c, int32_t bits) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (bits != 32 && bits != 64) return fail(SES3D_HIP_EINVAL, "kernel precision must be 32 or 64 bits");
+    if (bits == c->kernel_bits) return 0;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->kernel_bits = bits;
+    c->initialised = false;   // init zeroes the accumulators and passes the type to the kernels
     return 0;
 }
 

@@ -77,6 +77,7 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_get_saving_vector": ([P, ip], C.c_int),
         "ses3d_hip_set_history_degree": ([P, i32], C.c_int),
         "ses3d_hip_history_size": ([P, C.POINTER(C.c_int64)], C.c_int),
+        "ses3d_hip_set_kernel_precision": ([P, i32], C.c_int),
         "ses3d_hip_forward_checkpointed": ([P, i32], C.c_int),
         "ses3d_hip_adjoint_checkpointed": ([P, P], C.c_int),
         "ses3d_hip_memory_bytes": ([P, C.POINTER(C.c_int64)], C.c_int),
@@ -299,6 +300,13 @@ class Solver:
         and the adjoint context."""
         _check(self.lib.ses3d_hip_set_history_degree(self.h, int(deg)))
         self.hdeg = int(deg)
+
+    def set_kernel_precision(self, bits: int):
+        """Kernel accumulators of an adjoint context: 64 (float64, default) or
+        32 (float32 like the reference's grad_* arrays: its kernels bit for
+        bit).  Takes effect at the next init."""
+        _check(self.lib.ses3d_hip_set_kernel_precision(self.h, int(bits)))
+        self.kernel_bits = int(bits)
 
     def _history_shape(self):
         d = getattr(self, "hdeg", 4)
