@@ -64,7 +64,8 @@ class Problem:
 
     def __init__(self, run, data: Sequence[np.ndarray], pml: int = 2, ckpt: int = 100,
                  events_per_batch: Optional[int] = None, q_model: Optional[np.ndarray] = None,
-                 taper_width: float = 0.0, progress=None, history_degree: int = 4):
+                 taper_width: float = 0.0, progress=None, history_degree: int = 4,
+                 kernel_bits: int = 64):
         self.run = run
         self.data = [np.asarray(d, np.float32) for d in data]
         self.pml = pml
@@ -82,6 +83,8 @@ class Problem:
         # polynomial degree of the stored forward fields (4: as the reference
         # stores them; lower: (d+1)^3/125 of the segment memory, lossy)
         self.history_degree = history_degree
+        # kernel accumulators: 64 (float64) or 32 (float32, as the reference)
+        self.kernel_bits = kernel_bits
 
     def fields(self, rho, vp, vsh, vsv, Q=None) -> dict:
         rho, vp, vsh, vsv = (np.asarray(a, np.float64) for a in (rho, vp, vsh, vsv))
@@ -100,6 +103,8 @@ class Problem:
         s = Solver.from_run(self.run, adjoint_flag=af, pml=self.pml, n_events=n)
         if af:
             s.set_history_degree(self.history_degree)
+        if af == 2:
+            s.set_kernel_precision(self.kernel_bits)
         s.set_model(model)
         return s
 

@@ -208,6 +208,8 @@ program ses3d_hip_main
         end do
         call hip_check(ses3d_hip_set_adjoint_sources(ctx, 0_c_int32_t, int(nad, c_int32_t), adloc, adstf), &
              'set_adjoint_sources')
+        ! float32 kernel sums, as the reference's grad_* arrays: its kernel files bit for bit
+        call hip_check(ses3d_hip_set_kernel_precision(ctx, 32_c_int32_t), 'set_kernel_precision')
      end if
      call hip_check(ses3d_hip_init(ctx), 'init')
 
@@ -299,7 +301,7 @@ program ses3d_hip_main
         deallocate(sv, hist)
      end if
      if (adjoint_flag == 2) then
-        ! kernels (SOURCE/ses3d_output.f90:318-343), float64 accumulators written as float32
+        ! kernels (SOURCE/ses3d_output.f90:318-343): float32 sums, returned widened, written as float32
         allocate(g1(npts), g2(npts), g3(npts), g4(npts), field(npts))
         call hip_check(ses3d_hip_get_kernels(ctx, 0_c_int32_t, g1, g2, g3, g4, c_null_ptr), 'get_kernels')
         field = real(g1); call write_field(trim(ofd)//'grad_rho_0', field)

@@ -145,6 +145,13 @@ module ses3d_hip_mod
        integer(c_int32_t), value :: deg
      end function
 
+     ! type of the kernel accumulators: 32 (float32, the reference's grad_* arrays) or 64
+     integer(c_int) function ses3d_hip_set_kernel_precision(ctx, bits) bind(c, name='ses3d_hip_set_kernel_precision')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: bits
+     end function
+
      integer(c_int) function ses3d_hip_history_size(ctx, n) bind(c, name='ses3d_hip_history_size')
        import :: c_int, c_ptr, c_int64_t
        type(c_ptr), value :: ctx

@@ -54,8 +54,8 @@ def test_driver_outputs_match_reference(name, tmp_path):
                                           fx[f"history_{tag}"][k])
     FC.write_adjoint(fx, root, fx["seismograms"])
     _run(root)
+    # float32 kernel sums as the reference's: its kernel files bit for bit
     for g in ("grad_rho", "grad_cp", "grad_csh", "grad_csv"):
-        k = F.read_field(os.path.join(ofd, f"{g}_0"), shape).astype(np.float64)
-        ref = fx[g].astype(np.float64)
-        rel = np.linalg.norm(k - ref) / np.linalg.norm(ref)
-        assert rel <= 1e-5, (g, rel)
+        k = F.read_field(os.path.join(ofd, f"{g}_0"), shape)
+        assert np.linalg.norm(fx[g].astype(np.float64)) > 0, g
+        np.testing.assert_array_equal(k, fx[g], err_msg=g)

@@ -159,3 +159,39 @@ def test_kernel_precision_rejects_other_types(tmp_path):
         with pytest.raises(Ses3dError, match="precision"):
             a.set_kernel_precision(bits)
     a.close()
+
+
+def test_autograd_problem_float32_kernels():
+    """Problem(kernel_bits=32): the misfit is unchanged and the gradient moves
+    by float32 accumulation rounding only."""
+    import torch
+    from dataclasses import replace
+    from ses3d_amd import configs
+    from ses3d_amd.autograd import Problem, misfit
+    from ses3d_amd.solver import Solver
+    w = configs.Workload("kb", (10, 10, 5), True, 2, 0.25, 240, 16, "blobs", period=25.0)
+    true = configs.build(w, seed=2)
+    cur = configs.build(replace(w, perturbation="none"), seed=2)
+    s = Solver.from_run(true, adjoint_flag=0, pml=2)
+    s.set_model(true.model)
+    s.load_event(0, true.events[0])
+    s.init()
+    s.run()
+    data = [s.seismograms()]
+    s.close()
+    m = cur.model
+    rho0 = 1.0 / m["rhoinv"].astype(np.float64)
+    vs0 = np.sqrt(m["mu"] / rho0)
+    vp0 = np.sqrt((m["lambda"] + 2 * m["mu"]) / rho0)
+    out = {}
+    for bits in (64, 32):
+        prob = Problem(cur, data, pml=2, ckpt=40, taper_width=20.0, kernel_bits=bits)
+        x = [torch.tensor(a, requires_grad=True) for a in (rho0, vp0, vs0, vs0.copy())]
+        chi = misfit(prob, *x)
+        chi.backward()
+        out[bits] = (float(chi.detach()), np.concatenate([t.grad.numpy().ravel() for t in x]))
+    assert out[32][0] == out[64][0]
+    g32, g64 = out[32][1], out[64][1]
+    rel = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+    print(f"gradient, float32 vs float64 kernel accumulators: rel L2 {rel:.2e}", flush=True)
+    assert np.linalg.norm(g64) > 0 and rel <= 1e-4
