@@ -129,7 +129,8 @@ def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
                        f"({times[1]:.1f} s and {times[0]:.1f} s)")}
 
 
-def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int = 4, kernel_bits: int = 64) -> dict:
+def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int = 4, kernel_bits: int = 64,
+                 keep: int = 1) -> dict:
     """Seconds per event-gradient on the differentiation workload (48x48x24
     anelastic, dt=0.15): data from the 'true' model (1D + 2% Gaussian blobs),
     gradient of the L2 waveform misfit at the 1D model by the checkpointed
@@ -140,7 +141,9 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int =
     history_degree < 4 stores the sampled forward fields at that polynomial
     degree (ses3d_store's `deg`; lossy, smaller segment buffer); kernel_bits=32
     accumulates the kernels in float32 as the reference does (its kernels bit
-    for bit) instead of float64."""
+    for bit) instead of float64; keep > 1 keeps the sampled steps of the last
+    `keep` segments of the checkpointed forward in HBM, so the adjoint replays
+    (keep-1)*k fewer forward steps."""
     from dataclasses import replace
     from ses3d_amd import configs, misfit
     from ses3d_amd.solver import Solver
@@ -170,7 +173,7 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int =
     dt = float(cur.events[0].dt)
     taper = misfit.cosine_taper(nt, dt, 40.0)
     t0 = time.perf_counter()
-    f.forward_checkpointed(k)
+    f.forward_checkpointed(k, keep=keep)
     chi = 0.0
     for j in range(n_events):
         u = f.seismograms(j)
@@ -190,6 +193,7 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int =
     if history_degree < 4:
         out["history_degree"] = history_degree
     out["kernel_accumulators"] = f"float{kernel_bits}"
+    out["kept_segments"] = keep
     f.close()
     a.close()
     return out
@@ -464,6 +468,11 @@ def main():
         g32 = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree, kernel_bits=32)
         g32["speedup_over_float64"] = out["gradient"]["s_per_event_gradient"] / g32["s_per_event_gradient"]
         out["gradient_f32"] = g32
+        # 8 of the 30 segments kept: a 199 GB sampled-step buffer, 700 fewer replayed steps
+        print("# gradient leg, float32 accumulators, 8 kept segments", file=sys.stderr, flush=True)
+        g8 = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree, kernel_bits=32, keep=8)
+        g8["speedup_over_float64"] = out["gradient"]["s_per_event_gradient"] / g8["s_per_event_gradient"]
+        out["gradient_f32_keep8"] = g8
     if not args.no_batched:
         print("# batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg()

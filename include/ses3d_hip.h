@@ -196,6 +196,15 @@ int ses3d_hip_set_kernel_precision(ses3d_hip_context *ctx, int32_t bits);
 int ses3d_hip_forward_checkpointed(ses3d_hip_context *fwd, int32_t k_ckpt);
 int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *adj, ses3d_hip_context *fwd);
 
+/* Memory for replay: the next forward_checkpointed keeps the sampled steps of
+ * its last `segments` segments (default 1) in HBM, so the adjoint replays
+ * only the earlier ones: (segments-1)*k_ckpt fewer forward steps for a
+ * sampled-step buffer `segments` times larger (9 floats per GLL point per
+ * sampled step).  Kernels are identical for every value.  No counterpart in
+ * the reference, which keeps every sampled step on disk
+ * (SOURCE/ses3d_output.f90:5-267). */
+int ses3d_hip_set_checkpoint_keep(ses3d_hip_context *fwd, int32_t segments);
+
 /* Launch structure: the step number lives in HBM and each step is a fixed
  * sequence of kernels, so `steps` consecutive steps are captured once into a
  * hipGraph and replayed (default 16; 0 launches every kernel directly).  No

@@ -1233,6 +1233,8 @@ struct ses3d_hip_context : StepRunner {
     size_t recs_cap = 0, groups_cap = 0, forces_cap = 0;
     std::vector<int> saving_vector;   // 1..nt
     int ckpt_k = 0;
+    int ckpt_keep = 1;       // trailing segments whose sampled steps the checkpointed forward keeps
+    int ckpt_keep_used = 0;  // min(ckpt_keep, segments) of the last checkpointed forward
     bool ckpt_ready = false;
     bool last_seg_ready = false;   // the segment buffer holds the last segment's sampled steps
     // step number in HBM
@@ -1969,7 +1971,9 @@ __global__ void k_project(const float *__restrict__ u, int NX, int NY, long long
 
 // Segment buffer of every event: the sampled steps of the longest segment of
 // k steps.  Table of one segment's sampled steps into it.
-int ensure_segment_buffer(ses3d_hip_context *f, int k) {
+// Sampled-step buffer of a checkpointed forward: one replayed segment, or the
+// `keep` trailing segments the forward pass stores directly, whichever is larger.
+int ensure_segment_buffer(ses3d_hip_context *f, int k, int keep) {
     const int nt = f->cfg.nt, nseg = (nt + k - 1) / k;
     int max_slots = 0;
     const EventHost &E0 = f->evh[0];
@@ -1978,6 +1982,9 @@ int ensure_segment_buffer(ses3d_hip_context *f, int k) {
         for (int it = sgi * k + 1; it <= std::min((sgi + 1) * k, nt); ++it) n += E0.slot_h[it] >= 0;
         max_slots = std::max(max_slots, n);
     }
+    int tail = 0;
+    for (int it = (nseg - keep) * k + 1; it <= nt; ++it) tail += E0.slot_h[it] >= 0;
+    max_slots = std::max(max_slots, tail);
     for (auto &E : f->evh)
         if (E.seg_slots < max_slots) {
             if (E.seg) {
@@ -2664,6 +2671,8 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
     if (k < 1) return fail(SES3D_HIP_EINVAL, "checkpoint interval must be >= 1");
     const int nt = c->cfg.nt;
     const int nseg = (nt + k - 1) / k;
+    const int keep = std::min(c->ckpt_keep, nseg);
+    const int first_kept = nseg - keep;   // the first segment whose sampled steps are kept
     const size_t NP = (size_t)c->NP, NGB = (size_t)c->NGB;
     const size_t nM = c->cfg.is_diss ? 18 * NP : 0;
     const size_t ck = 3 * NGB + 18 * NP + nM;
@@ -2679,19 +2688,22 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
         TRY(upload_slots(c, E, std::vector<int>(nt + 1, -1)));
     }
     TRY(refresh_events(c));
-    TRY(ensure_segment_buffer(c, k));
+    TRY(ensure_segment_buffer(c, k, keep));
     c->ckpt_k = k;
+    c->ckpt_keep_used = keep;
     c->last_seg_ready = false;
     for (int sgi = 0; sgi < nseg; ++sgi) {
-        if (sgi == nseg - 1) {
-            // the last segment stores its sampled steps right away: the adjoint
-            // starts with it and need not replay it
+        if (sgi == first_kept) {
+            // the trailing segments store their sampled steps right away: the
+            // adjoint starts with them and need not replay them
             for (auto &E : c->evh) {
                 E.hist = E.seg;
                 TRY(upload_slots(c, E, segment_table(E, nt, sgi * k + 1, nt)));
             }
             TRY(refresh_events(c));
         }
+        // restart state of every segment (a kept one is replayed only by a
+        // second adjoint, after the first overwrote the kept steps)
         for (auto &E : c->evh) {
             float *dst = E.ckpt[sgi];
             HIP_TRY(hipMemcpyAsync(dst, E.v, sizeof(float) * 3 * NGB, hipMemcpyDeviceToDevice, c->stream));
@@ -2708,6 +2720,13 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
     return 0;
 }
 
+int ses3d_hip_set_checkpoint_keep(ses3d_hip_context *c, int32_t segments) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (segments < 1) return fail(SES3D_HIP_EINVAL, "kept segments must be >= 1");
+    c->ckpt_keep = segments;
+    return 0;
+}
+
 int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
     if (!a || !f) return fail(SES3D_HIP_EINVAL, "null context");
     if (!a->initialised) return fail(SES3D_HIP_ESTATE, "init the adjoint context first");
@@ -2718,14 +2737,14 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
     if (a->hdeg != f->hdeg) return fail(SES3D_HIP_EINVAL, "forward and adjoint contexts store fields of different degree");
     const int nt = f->cfg.nt, k = f->ckpt_k;
     const int nseg = (nt + k - 1) / k;
+    const int first_kept = nseg - f->ckpt_keep_used;
     const size_t NP = (size_t)f->NP, NGB = (size_t)f->NGB;
     const size_t nM = f->cfg.is_diss ? 18 * NP : 0;
-    TRY(ensure_segment_buffer(f, k));
+    TRY(ensure_segment_buffer(f, k, f->ckpt_keep_used));
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
         const int s0 = sgi * k + 1, s1 = std::min((sgi + 1) * k, nt);
-        if (sgi == nseg - 1 && f->last_seg_ready) {
-            // stored by forward_checkpointed; the replays below overwrite it
-            f->last_seg_ready = false;
+        if (sgi >= first_kept && f->last_seg_ready) {
+            // stored by forward_checkpointed; the first replay below overwrites it
         } else {
             // restore the forward state at the start of the segment, replay it into the buffer
             f->last_seg_ready = false;

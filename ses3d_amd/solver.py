@@ -80,6 +80,7 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_set_kernel_precision": ([P, i32], C.c_int),
         "ses3d_hip_forward_checkpointed": ([P, i32], C.c_int),
         "ses3d_hip_adjoint_checkpointed": ([P, P], C.c_int),
+        "ses3d_hip_set_checkpoint_keep": ([P, i32], C.c_int),
         "ses3d_hip_memory_bytes": ([P, C.POINTER(C.c_int64)], C.c_int),
         "ses3d_hip_set_profiling": ([P, i32], C.c_int),
         "ses3d_hip_set_graph_steps": ([P, i32], C.c_int),
@@ -336,7 +337,11 @@ class Solver:
         _check(self.lib.ses3d_hip_get_saving_vector(self.h, out.ctypes.data_as(C.POINTER(C.c_int32))))
         return out
 
-    def forward_checkpointed(self, k: int):
+    def forward_checkpointed(self, k: int, keep: int = 1):
+        """Forward pass with restart states every k steps; the sampled steps of
+        the last ``keep`` segments stay in HBM, so the adjoint replays only the
+        earlier segments (kernels identical for every ``keep``)."""
+        _check(self.lib.ses3d_hip_set_checkpoint_keep(self.h, int(keep)))
         _check(self.lib.ses3d_hip_forward_checkpointed(self.h, int(k)))
         self.synchronize()
 

@@ -92,6 +92,46 @@ def test_checkpointed_gradient(tmp_path, is_diss, k):
             assert rel <= tol, (g, prec, rel)
 
 
+@pytest.mark.parametrize("keep", [2, 3, 9, 50])
+def test_checkpointed_gradient_kept_segments(tmp_path, keep):
+    """forward_checkpointed(k, keep): the sampled steps of the last `keep`
+    segments stay in HBM and the adjoint replays only the earlier segments
+    (nt=60, k=7: 9 segments; keep=9 and 50 keep them all, no replay).  The
+    kernels equal the stored-history kernels bit for bit, and so do those of a
+    second adjoint on the same forward, which replays every segment."""
+    run = make_case(str(tmp_path), nel=(6, 6, 4), nt=60, is_diss=True, adjoint_flag=1, samp_ad=4, seed=14)
+    e = run.events[0]
+    print(f"kept segments {keep}", flush=True)
+    f = _ctx(run, 1)
+    f.load_event(0, e)
+    f.init()
+    f.run()
+    loc, rows = _adjoint_sources(e, f.seismograms())
+    a = _ctx(run, 2)
+    a.set_adjoint_sources(0, loc, rows)
+    a.init()
+    for st in np.flatnonzero(f.saving_vector()) + 1:
+        a.put_history(int(st), f.history(int(st)))
+    a.run()
+    K1 = a.kernels()
+    f2 = _ctx(run, 1)
+    f2.load_event(0, e)
+    f2.init()
+    f2.forward_checkpointed(7, keep=keep)
+    for rep in range(2):
+        a2 = _ctx(run, 2)
+        a2.set_adjoint_sources(0, loc, rows)
+        a2.init()
+        a2.adjoint_checkpointed(f2)
+        K2 = a2.kernels()
+        a2.close()
+        for g in K1:
+            assert np.linalg.norm(K1[g]) > 0, g
+            np.testing.assert_array_equal(K2[g], K1[g], err_msg=f"adjoint {rep + 1}: {g}")
+    for s in (f, a, f2):
+        s.close()
+
+
 def test_anisotropic_adjoint(tmp_path):
     """Adjoint run of an anisotropic anelastic model (nonzero A, B, C): the
     adjoint wavefield equals the float32 oracle's bit for bit and the kernels
