@@ -65,11 +65,13 @@ class Problem:
     def __init__(self, run, data: Sequence[np.ndarray], pml: int = 2, ckpt: int = 100,
                  events_per_batch: Optional[int] = None, q_model: Optional[np.ndarray] = None,
                  taper_width: float = 0.0, progress=None, history_degree: int = 4,
-                 kernel_bits: int = 64):
+                 kernel_bits: int = 64, ckpt_keep: int = 1):
         self.run = run
         self.data = [np.asarray(d, np.float32) for d in data]
         self.pml = pml
         self.ckpt = ckpt
+        # trailing checkpoint segments whose sampled steps stay in HBM (no replay)
+        self.ckpt_keep = ckpt_keep
         self.n_events = len(run.events)
         self.events_per_batch = events_per_batch or self.n_events
         self.W = quadrature_weights(run)
@@ -152,7 +154,7 @@ class Problem:
                 for k, ev in enumerate(evs):
                     f.load_event(k, self.run.events[ev])
                 f.init()
-                f.forward_checkpointed(self.ckpt)
+                f.forward_checkpointed(self.ckpt, keep=self.ckpt_keep)
                 for k, ev in enumerate(evs):
                     u = f.seismograms(k)
                     self.last_seismograms.append(u)

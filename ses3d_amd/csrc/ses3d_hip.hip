@@ -1236,7 +1236,7 @@ struct ses3d_hip_context : StepRunner {
     int ckpt_keep = 1;       // trailing segments whose sampled steps the checkpointed forward keeps
     int ckpt_keep_used = 0;  // min(ckpt_keep, segments) of the last checkpointed forward
     bool ckpt_ready = false;
-    bool last_seg_ready = false;   // the segment buffer holds the last segment's sampled steps
+    bool last_seg_ready = false;   // the segment buffer holds the kept trailing segments' sampled steps
     // step number in HBM
     int *it_d = nullptr;
     unsigned long long div_bad[4] = {0, 0, 0, 0};   // numerators failing the reciprocal check
@@ -1969,10 +1969,10 @@ __global__ void k_project(const float *__restrict__ u, int NX, int NY, long long
     out[t] = acc;
 }
 
-// Segment buffer of every event: the sampled steps of the longest segment of
-// k steps.  Table of one segment's sampled steps into it.
-// Sampled-step buffer of a checkpointed forward: one replayed segment, or the
-// `keep` trailing segments the forward pass stores directly, whichever is larger.
+// Sampled-step buffer of every event for a checkpointed forward: the sampled
+// steps of the longest segment of k steps, or of the `keep` trailing segments
+// the forward pass stores directly, whichever is more (segment_table maps a
+// range of steps into it).
 int ensure_segment_buffer(ses3d_hip_context *f, int k, int keep) {
     const int nt = f->cfg.nt, nseg = (nt + k - 1) / k;
     int max_slots = 0;

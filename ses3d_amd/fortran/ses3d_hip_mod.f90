@@ -152,6 +152,25 @@ module ses3d_hip_mod
        integer(c_int32_t), value :: bits
      end function
 
+     ! gradient without disk history: restart states in HBM every k steps, the
+     ! sampled steps of the last `segments` segments kept, the rest replayed
+     integer(c_int) function ses3d_hip_set_checkpoint_keep(fwd, segments) bind(c, name='ses3d_hip_set_checkpoint_keep')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: fwd
+       integer(c_int32_t), value :: segments
+     end function
+
+     integer(c_int) function ses3d_hip_forward_checkpointed(fwd, k) bind(c, name='ses3d_hip_forward_checkpointed')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: fwd
+       integer(c_int32_t), value :: k
+     end function
+
+     integer(c_int) function ses3d_hip_adjoint_checkpointed(adj, fwd) bind(c, name='ses3d_hip_adjoint_checkpointed')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: adj, fwd
+     end function
+
      integer(c_int) function ses3d_hip_history_size(ctx, n) bind(c, name='ses3d_hip_history_size')
        import :: c_int, c_ptr, c_int64_t
        type(c_ptr), value :: ctx

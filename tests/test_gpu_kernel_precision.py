@@ -163,7 +163,8 @@ def test_kernel_precision_rejects_other_types(tmp_path):
 
 def test_autograd_problem_float32_kernels():
     """Problem(kernel_bits=32): the misfit is unchanged and the gradient moves
-    by float32 accumulation rounding only."""
+    by float32 accumulation rounding only; Problem(ckpt_keep=3) gives the same
+    gradient bit for bit."""
     import torch
     from dataclasses import replace
     from ses3d_amd import configs
@@ -184,12 +185,16 @@ def test_autograd_problem_float32_kernels():
     vs0 = np.sqrt(m["mu"] / rho0)
     vp0 = np.sqrt((m["lambda"] + 2 * m["mu"]) / rho0)
     out = {}
-    for bits in (64, 32):
-        prob = Problem(cur, data, pml=2, ckpt=40, taper_width=20.0, kernel_bits=bits)
+    for bits, keep in ((64, 1), (32, 1), (32, 3)):
+        prob = Problem(cur, data, pml=2, ckpt=40, taper_width=20.0, kernel_bits=bits, ckpt_keep=keep)
         x = [torch.tensor(a, requires_grad=True) for a in (rho0, vp0, vs0, vs0.copy())]
         chi = misfit(prob, *x)
         chi.backward()
-        out[bits] = (float(chi.detach()), np.concatenate([t.grad.numpy().ravel() for t in x]))
+        out[bits, keep] = (float(chi.detach()), np.concatenate([t.grad.numpy().ravel() for t in x]))
+    # kept checkpoint segments change nothing
+    assert out[32, 3][0] == out[32, 1][0]
+    np.testing.assert_array_equal(out[32, 3][1], out[32, 1][1])
+    out = {bits: out[bits, 1] for bits in (64, 32)}
     assert out[32][0] == out[64][0]
     g32, g64 = out[32][1], out[64][1]
     rel = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
