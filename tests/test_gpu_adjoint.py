@@ -132,6 +132,46 @@ def test_checkpointed_gradient_kept_segments(tmp_path, keep):
         s.close()
 
 
+def test_kept_segments_two_events_lower_degree(tmp_path):
+    """Kept segments in a two-event context whose sampled steps are stored at
+    degree 2: the kernels of every event equal those of keep=1 bit for bit."""
+    from ses3d_amd.solver import Solver
+    run = make_case(str(tmp_path), nel=(5, 6, 4), nt=45, is_diss=True, adjoint_flag=1, samp_ad=2, seed=31,
+                    n_events=2)
+
+    def ctx(af):
+        s = Solver.from_run(run, adjoint_flag=af)
+        s.set_history_degree(2)
+        s.set_model(run.model)
+        return s
+
+    def kernels(keep):
+        f = ctx(1)
+        for k, e in enumerate(run.events):
+            f.load_event(k, e)
+        f.init()
+        f.forward_checkpointed(6, keep=keep)
+        a = ctx(2)
+        for k, e in enumerate(run.events):
+            loc, rows = _adjoint_sources(e, f.seismograms(k))
+            a.set_adjoint_sources(k, e.rec32[f.receiver_index(k)], rows)
+        a.init()
+        a.adjoint_checkpointed(f)
+        out = [a.kernels(k) for k in range(2)]
+        f.close()
+        a.close()
+        return out
+
+    print("two events, degree 2, keep 1 vs 4 vs all", flush=True)
+    K1 = kernels(1)
+    for keep in (4, 8):
+        Kk = kernels(keep)
+        for ev in range(2):
+            for g in K1[ev]:
+                assert ev > 0 or np.linalg.norm(K1[ev][g]) > 0, g
+                np.testing.assert_array_equal(Kk[ev][g], K1[ev][g], err_msg=f"keep {keep} event {ev} {g}")
+
+
 def test_anisotropic_adjoint(tmp_path):
     """Adjoint run of an anisotropic anelastic model (nonzero A, B, C): the
     adjoint wavefield equals the float32 oracle's bit for bit and the kernels
