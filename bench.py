@@ -357,6 +357,9 @@ def main():
     ap.add_argument("--no-elastic", action="store_true", help="skip the elastic 24x24x12 leg")
     ap.add_argument("--checkpoint-every", type=int, default=100,
                     help="restart states every k steps in the gradient legs")
+    ap.add_argument("--checkpoint-keep", type=int, default=1,
+                    help="trailing checkpoint segments whose sampled steps stay in HBM (no replay) in the "
+                         "plain and float32 gradient legs")
     ap.add_argument("--history-degree", type=int, default=4,
                     help="polynomial degree of the stored forward fields in the gradient legs (4: unchanged)")
     ap.add_argument("--gradient-events", type=int, default=4,
@@ -459,13 +462,15 @@ def main():
         out["elastic_24x24x12"] = small_forward_leg()
     if not args.no_gradient:
         print("# gradient leg", file=sys.stderr, flush=True)
-        out["gradient"] = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree)
+        out["gradient"] = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree,
+                                       keep=args.checkpoint_keep)
         print("# batched gradient leg", file=sys.stderr, flush=True)
         gb = gradient_leg(k=args.checkpoint_every, n_events=args.gradient_events, history_degree=args.history_degree)
         gb["per_event_speedup_over_plain"] = out["gradient"]["s_per_event_gradient"] / gb["s_per_event_gradient"]
         out["gradient_batched"] = gb
         print("# gradient leg, float32 kernel accumulators", file=sys.stderr, flush=True)
-        g32 = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree, kernel_bits=32)
+        g32 = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree, kernel_bits=32,
+                           keep=args.checkpoint_keep)
         g32["speedup_over_float64"] = out["gradient"]["s_per_event_gradient"] / g32["s_per_event_gradient"]
         out["gradient_f32"] = g32
         # 8 of the 30 segments kept: a 199 GB sampled-step buffer, 700 fewer replayed steps
