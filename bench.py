@@ -298,7 +298,7 @@ def inversion_leg(n_events: int = 16, nt: int = 4000, k: int = 100, step_frac: f
     (checkpointed forward every k steps + adjoint with float64 kernel
     accumulation, one event per solve: the checkpoints of one event take ~160 GB),
     a steepest-descent update of vp, vsh, vsv (step_frac of their means along the
-    max-normalised gradient), and the misfit before and after it judged by plain
+    max-normalised gradient), and the misfit before and after it, measured by plain
     single-event forward solves."""
     import torch
     from dataclasses import replace

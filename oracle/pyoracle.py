@@ -254,7 +254,8 @@ def ref_available() -> bool:
 
 def build_ref() -> bool:
     """Build oracle/_ref/ses3d.exe from the reference sources where they lie.
-    Returns False when the reference tree is absent (e.g. on a GPU box)."""
+    Returns False when the reference tree is absent (a machine that received
+    only this repository)."""
     if not os.path.isdir("/root/reference/SOURCE"):
         return ref_available()
     r = subprocess.run(["make", "-s", "-C", HERE, "ref"], capture_output=True, text=True)

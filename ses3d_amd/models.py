@@ -338,7 +338,7 @@ def decompose(setup: F.Setup, px: int, py: int, pz: int) -> F.Setup:
 def section_setup(n_theta: int, n_phi: int, n_r: int, is_diss: bool = False,
                   adjoint_flag: int = 0, samp_ad: int = 15,
                   ffd: str = "../DATA/OUTPUT/FWD") -> F.Setup:
-    """The BASELINE spherical section (colat 40-60, lon 10-30, r 5371-6371 km)
+    """The benchmark workloads' spherical section (colat 40-60, lon 10-30, r 5371-6371 km)
     with n_theta x n_phi x n_r elements on one rank box."""
     return F.Setup(40.0, 60.0, 10.0, 30.0, 5371000.0, 6371000.0, int(is_diss), 1,
                    n_theta - 1, n_phi - 1, n_r - 1, 4, 1, 1, 1, adjoint_flag, samp_ad, ffd)

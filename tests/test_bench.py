@@ -1,6 +1,6 @@
 """The benchmark's contract pieces that need no GPU: the refusal of a
-multi-GPU run and the algorithmic byte counts the roofline uses (the survey's
-per-point figures: 178.9 B elastic at 24x24x12 and 326.5 B anelastic per whole
+multi-GPU run and the algorithmic byte counts the roofline uses (the per-point
+figures: 178.9 B elastic at 24x24x12 and 326.5 B anelastic per whole
 step; the element kernel's share without the velocity write and mass read)."""
 import os
 import subprocess

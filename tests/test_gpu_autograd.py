@@ -91,7 +91,7 @@ def test_anelastic_gradient_vs_finite_differences():
 
 def test_steepest_descent_update_lowers_misfit():
     """One steepest-descent model update from the gradient of two events solved
-    together in one context lowers the misfit, judged by plain forward solves of
+    together in one context lowers the misfit, checked by plain forward solves of
     one event at a time (the acceptance check of an inversion iteration, at a
     small size)."""
     from ses3d_amd.autograd import Problem, misfit
@@ -120,10 +120,10 @@ def test_steepest_descent_update_lowers_misfit():
     # step: 0.5% of the velocities along the steepest descent of vp, vsh, vsv
     scale = [0.0, 0.005 * vp0.mean(), 0.005 * vs0.mean(), 0.005 * vs0.mean()]
     new = [x[k].detach().numpy() - scale[k] * g[k] / max(np.abs(g[k]).max(), 1e-300) for k in range(4)]
-    # judge with plain single-event solves
-    judge = Problem(cur, data, pml=2, ckpt=40, taper_width=20.0, events_per_batch=1)
-    chi_before = judge.misfit(judge.fields(rho0, vp0, vs0, vs0))
-    chi_after = judge.misfit(judge.fields(*new))
+    # check with plain single-event solves
+    check = Problem(cur, data, pml=2, ckpt=40, taper_width=20.0, events_per_batch=1)
+    chi_before = check.misfit(check.fields(rho0, vp0, vs0, vs0))
+    chi_after = check.misfit(check.fields(*new))
     print(f"misfit before {chi_before:.6e} after {chi_after:.6e}", flush=True)
     assert abs(chi_before - float(chi0.detach())) <= 1e-6 * chi_before
     assert chi_after < chi_before
