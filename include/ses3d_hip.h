@@ -205,6 +205,16 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *adj, ses3d_hip_context *fw
  * (SOURCE/ses3d_output.f90:5-267). */
 int ses3d_hip_set_checkpoint_keep(ses3d_hip_context *fwd, int32_t segments);
 
+/* Set-up half of ses3d_hip_forward_checkpointed: allocate (and clear) the
+ * restart states for interval k_ckpt and the sampled-step buffer for the
+ * current kept-segment count, so that the next forward_checkpointed with the
+ * same k_ckpt allocates nothing.  Call after init; a checkpointed forward
+ * run before it is discarded (adjoint_checkpointed then refuses until the
+ * next forward_checkpointed).  Optional: forward_checkpointed allocates what
+ * is missing itself.  Replaces the reference's up-front sizing of its module
+ * arrays (SOURCE/ses3d_modules.f90:33-233), which allocates nothing per run. */
+int ses3d_hip_prepare_checkpoints(ses3d_hip_context *fwd, int32_t k_ckpt);
+
 /* Launch structure: the step number lives in HBM and each step is a fixed
  * sequence of kernels, so `steps` consecutive steps are captured once into a
  * hipGraph and replayed (default 16; 0 launches every kernel directly).  No

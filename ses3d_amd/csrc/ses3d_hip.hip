@@ -2005,6 +2005,23 @@ std::vector<int> segment_table(const EventHost &E, int nt, int s0, int s1) {
     return tab;
 }
 
+// Restart states (one per segment of k steps) of every event and the
+// sampled-step buffer for `keep` kept segments; buffers already large enough
+// are kept, so a repeated checkpointed forward allocates nothing.
+int ensure_checkpoints(ses3d_hip_context *c, int k, int keep) {
+    const int nseg = (c->cfg.nt + k - 1) / k;
+    const size_t NP = (size_t)c->NP, NGB = (size_t)c->NGB;
+    const size_t ck = 3 * NGB + 18 * NP + (c->cfg.is_diss ? 18 * NP : 0);
+    for (auto &E : c->evh)
+        if ((int)E.ckpt.size() != nseg) {
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            for (float *q : E.ckpt) dfree(c, q);
+            E.ckpt.assign(nseg, nullptr);
+            for (int sgi = 0; sgi < nseg; ++sgi) TRY(dalloc(c, &E.ckpt[sgi], ck));
+        }
+    return ensure_segment_buffer(c, k, keep);
+}
+
 }  // namespace
 
 // ============================================================================
@@ -2675,20 +2692,13 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
     const int first_kept = nseg - keep;   // the first segment whose sampled steps are kept
     const size_t NP = (size_t)c->NP, NGB = (size_t)c->NGB;
     const size_t nM = c->cfg.is_diss ? 18 * NP : 0;
-    const size_t ck = 3 * NGB + 18 * NP + nM;
+    TRY(ensure_checkpoints(c, k, keep));
     // the forward pass keeps the displacement gradient (mode 1) but stores no sampled step
     for (auto &E : c->evh) {
-        if ((int)E.ckpt.size() != nseg) {
-            HIP_TRY(hipStreamSynchronize(c->stream));
-            for (float *q : E.ckpt) dfree(c, q);
-            E.ckpt.assign(nseg, nullptr);
-            for (int s = 0; s < nseg; ++s) TRY(dalloc(c, &E.ckpt[s], ck));
-        }
         E.hist = nullptr;
         TRY(upload_slots(c, E, std::vector<int>(nt + 1, -1)));
     }
     TRY(refresh_events(c));
-    TRY(ensure_segment_buffer(c, k, keep));
     c->ckpt_k = k;
     c->ckpt_keep_used = keep;
     c->last_seg_ready = false;
@@ -2724,6 +2734,28 @@ int ses3d_hip_set_checkpoint_keep(ses3d_hip_context *c, int32_t segments) {
     if (!c) return fail(SES3D_HIP_EINVAL, "null context");
     if (segments < 1) return fail(SES3D_HIP_EINVAL, "kept segments must be >= 1");
     c->ckpt_keep = segments;
+    return 0;
+}
+
+int ses3d_hip_prepare_checkpoints(ses3d_hip_context *c, int32_t k) {
+    if (!c) return fail(SES3D_HIP_EINVAL, "null context");
+    if (!c->initialised) return fail(SES3D_HIP_ESTATE, "init must precede prepare_checkpoints");
+    if (c->cfg.adjoint_flag != 1) return fail(SES3D_HIP_ESTATE, "prepare_checkpoints needs an adjoint_flag=1 context");
+    if (k < 1) return fail(SES3D_HIP_EINVAL, "checkpoint interval must be >= 1");
+    const int nseg = (c->cfg.nt + k - 1) / k;
+    TRY(ensure_checkpoints(c, k, std::min(c->ckpt_keep, nseg)));
+    // the buffers are cleared below: a previous checkpointed forward is gone
+    c->ckpt_ready = false;
+    c->last_seg_ready = false;
+    // first touch of every page here rather than in the forward pass
+    const size_t NP = (size_t)c->NP, NGB = (size_t)c->NGB;
+    const size_t ck = 3 * NGB + 18 * NP + (c->cfg.is_diss ? 18 * NP : 0);
+    for (auto &E : c->evh) {
+        for (float *q : E.ckpt) HIP_TRY(hipMemsetAsync(q, 0, sizeof(float) * ck, c->stream));
+        if (E.seg && E.seg_slots > 0)
+            HIP_TRY(hipMemsetAsync(E.seg, 0, sizeof(float) * (size_t)E.seg_slots * hist_slot_floats(c), c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -160,6 +160,13 @@ module ses3d_hip_mod
        integer(c_int32_t), value :: segments
      end function
 
+     ! optional: allocate the checkpoint storage before the timed forward pass
+     integer(c_int) function ses3d_hip_prepare_checkpoints(fwd, k) bind(c, name='ses3d_hip_prepare_checkpoints')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: fwd
+       integer(c_int32_t), value :: k
+     end function
+
      integer(c_int) function ses3d_hip_forward_checkpointed(fwd, k) bind(c, name='ses3d_hip_forward_checkpointed')
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: fwd

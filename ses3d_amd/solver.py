@@ -81,6 +81,7 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_forward_checkpointed": ([P, i32], C.c_int),
         "ses3d_hip_adjoint_checkpointed": ([P, P], C.c_int),
         "ses3d_hip_set_checkpoint_keep": ([P, i32], C.c_int),
+        "ses3d_hip_prepare_checkpoints": ([P, i32], C.c_int),
         "ses3d_hip_memory_bytes": ([P, C.POINTER(C.c_int64)], C.c_int),
         "ses3d_hip_set_profiling": ([P, i32], C.c_int),
         "ses3d_hip_set_graph_steps": ([P, i32], C.c_int),
@@ -344,6 +345,14 @@ class Solver:
         _check(self.lib.ses3d_hip_set_checkpoint_keep(self.h, int(keep)))
         _check(self.lib.ses3d_hip_forward_checkpointed(self.h, int(k)))
         self.synchronize()
+
+    def prepare_checkpoints(self, k: int, keep: int = 1):
+        """Allocate and clear the restart states for interval k and the
+        sampled-step buffer for ``keep`` kept segments ahead of
+        forward_checkpointed(k, keep), which then allocates nothing; discards
+        an earlier checkpointed forward."""
+        _check(self.lib.ses3d_hip_set_checkpoint_keep(self.h, int(keep)))
+        _check(self.lib.ses3d_hip_prepare_checkpoints(self.h, int(k)))
 
     def adjoint_checkpointed(self, fwd: "Solver"):
         _check(self.lib.ses3d_hip_adjoint_checkpointed(self.h, fwd.h))

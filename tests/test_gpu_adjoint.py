@@ -132,6 +132,53 @@ def test_checkpointed_gradient_kept_segments(tmp_path, keep):
         s.close()
 
 
+def test_prepared_checkpoints_allocate_nothing(tmp_path):
+    """prepare_checkpoints(k, keep) allocates the checkpoint storage ahead of
+    forward_checkpointed, which then allocates nothing; the kernels equal those
+    of an unprepared context bit for bit; a prepare between forward and adjoint
+    discards the checkpointed forward, so the adjoint refuses to run until the
+    forward is repeated (from a fresh init, which restarts the wave field and
+    keeps the prepared storage)."""
+    from ses3d_amd.solver import Ses3dError
+    run = make_case(str(tmp_path), nel=(6, 6, 4), nt=60, is_diss=True, adjoint_flag=1, samp_ad=4, seed=14)
+    e = run.events[0]
+
+    def gradient(prepare, keep):
+        f = _ctx(run, 1)
+        f.load_event(0, e)
+        f.init()
+        if prepare:
+            f.prepare_checkpoints(7, keep=keep)
+            b0 = f.memory_bytes()
+        f.forward_checkpointed(7, keep=keep)
+        if prepare:
+            assert f.memory_bytes() == b0
+        loc, rows = _adjoint_sources(e, f.seismograms())
+        a = _ctx(run, 2)
+        a.set_adjoint_sources(0, loc, rows)
+        a.init()
+        if prepare:
+            f.prepare_checkpoints(7, keep=keep)
+            with pytest.raises(Ses3dError, match="forward_checkpointed"):
+                a.adjoint_checkpointed(f)
+            f.init()
+            f.forward_checkpointed(7, keep=keep)
+            assert f.memory_bytes() == b0
+        a.adjoint_checkpointed(f)
+        K = a.kernels()
+        f.close()
+        a.close()
+        return K
+
+    for keep in (1, 3):
+        print(f"prepared checkpoints, keep {keep}", flush=True)
+        K0 = gradient(False, keep)
+        K1 = gradient(True, keep)
+        for g in K0:
+            assert np.linalg.norm(K0[g]) > 0, g
+            np.testing.assert_array_equal(K1[g], K0[g], err_msg=f"keep {keep} {g}")
+
+
 def test_kept_segments_two_events_lower_degree(tmp_path):
     """Kept segments in a two-event context whose sampled steps are stored at
     degree 2: the kernels of every event equal those of keep=1 bit for bit."""
