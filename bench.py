@@ -129,6 +129,11 @@ def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
                        f"({times[1]:.1f} s and {times[0]:.1f} s)")}
 
 
+def configs_nt(name: str) -> int:
+    from ses3d_amd import configs
+    return configs.WORKLOADS[name].nt
+
+
 def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int = 4, kernel_bits: int = 64,
                  keep: int = 1) -> dict:
     """Seconds per event-gradient on the differentiation workload (48x48x24
@@ -482,6 +487,14 @@ def main():
         g8 = gradient_leg(k=args.checkpoint_every, history_degree=args.history_degree, kernel_bits=32, keep=8)
         g8["speedup_over_float64"] = out["gradient"]["s_per_event_gradient"] / g8["s_per_event_gradient"]
         out["gradient_f32_keep8"] = g8
+        # stored fields at degree 2 (27 of 125 values per element, ses3d_store's deg=2): every
+        # segment's sampled steps fit in HBM (a 161 GB buffer), so nothing is replayed
+        print("# gradient leg, float32 accumulators, degree-2 stored fields, all segments kept",
+              file=sys.stderr, flush=True)
+        nseg = -(-configs_nt("gradient_48x48x24") // args.checkpoint_every)
+        gd = gradient_leg(k=args.checkpoint_every, history_degree=2, kernel_bits=32, keep=nseg)
+        gd["speedup_over_float64"] = out["gradient"]["s_per_event_gradient"] / gd["s_per_event_gradient"]
+        out["gradient_f32_deg2_keepall"] = gd
     if not args.no_batched:
         print("# batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg()
