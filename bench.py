@@ -135,7 +135,7 @@ def configs_nt(name: str) -> int:
 
 
 def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int = 4, kernel_bits: int = 64,
-                 keep: int = 1) -> dict:
+                 keep: int = 1, workload: str = "gradient_48x48x24") -> dict:
     """Seconds per event-gradient on the differentiation workload (48x48x24
     anelastic, dt=0.15): data from the 'true' model (1D + 2% Gaussian blobs),
     gradient of the L2 waveform misfit at the 1D model by the checkpointed
@@ -152,7 +152,7 @@ def gradient_leg(k: int = 100, nt=None, n_events: int = 1, history_degree: int =
     from dataclasses import replace
     from ses3d_amd import configs, misfit
     from ses3d_amd.solver import Solver
-    w = configs.WORKLOADS["gradient_48x48x24"]
+    w = configs.WORKLOADS[workload]
     nt = nt or w.nt
     true = configs.build(w, seed=0, nt=nt, n_events=n_events)
     cur = configs.build(replace(w, perturbation="none"), seed=0, nt=nt, n_events=n_events)
@@ -495,6 +495,14 @@ def main():
         gd = gradient_leg(k=args.checkpoint_every, history_degree=2, kernel_bits=32, keep=nseg)
         gd["speedup_over_float64"] = out["gradient"]["s_per_event_gradient"] / gd["s_per_event_gradient"]
         out["gradient_f32_deg2_keepall"] = gd
+        # where one event's step is launch-bound, batching pays: 32 events on 8x8x4 in one
+        # context against one event alone
+        print("# small-mesh gradient legs, plain and 32 events batched", file=sys.stderr, flush=True)
+        sp = gradient_leg(k=args.checkpoint_every, workload="gradient_8x8x4")
+        sb = gradient_leg(k=args.checkpoint_every, n_events=32, workload="gradient_8x8x4")
+        sb["per_event_speedup_over_plain"] = sp["s_per_event_gradient"] / sb["s_per_event_gradient"]
+        out["gradient_small_plain"] = sp
+        out["gradient_small_batched"] = sb
     if not args.no_batched:
         print("# batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg()
