@@ -76,4 +76,7 @@ def test_group_and_schedule_entry_points_reject_null():
     assert lib.ses3d_hip_group_get_streaming(None, None, None) != 0
     assert lib.ses3d_hip_set_kernel_precision(None, 32) != 0
     assert b"null" in lib.ses3d_hip_last_error()
+    assert lib.ses3d_hip_set_checkpoint_keep(None, 2) != 0
+    assert lib.ses3d_hip_prepare_checkpoints(None, 100) != 0
+    assert b"null" in lib.ses3d_hip_last_error()
     lib.ses3d_hip_group_destroy(None)
