@@ -843,12 +843,33 @@ __global__ void k_forces(const MeshDev *__restrict__ mesh, const EventDev *__res
     float *f = evs[G.ev].f;
     const long long p = G.elem * EST + fperm(nd % 5, (nd / 5) % 5, nd / 25);
     float fx = f[2 * p], fy = f[2 * p + 1], fz = f[2 * NP + p];
-    for (int s = G.first; s < G.first + G.count; ++s) {
-        const ForceDev F = forces[s];
-        const float w = F.w[nd];
-        if (F.comp_mask & 1) fx = fx - w * F.stf[0 * nt + it - 1];
-        if (F.comp_mask & 2) fy = fy - w * F.stf[1 * nt + it - 1];
-        if (F.comp_mask & 4) fz = fz - w * F.stf[2 * nt + it - 1];
+    // sources in chunks of FCH: the loads of a chunk are independent of each
+    // other (one round trip instead of one per source), the subtractions then
+    // run in the sources' order
+    constexpr int FCH = 8;
+    const int end = G.first + G.count;
+    for (int s0 = G.first; s0 < end; s0 += FCH) {
+        float w[FCH], sx[FCH], sy[FCH], sz[FCH];
+        int mask[FCH];
+#pragma unroll
+        for (int j = 0; j < FCH; ++j) {
+            mask[j] = 0;
+            w[j] = sx[j] = sy[j] = sz[j] = 0.0f;
+            if (s0 + j < end) {
+                const ForceDev F = forces[s0 + j];
+                mask[j] = F.comp_mask;
+                w[j] = F.w[nd];
+                if (F.comp_mask & 1) sx[j] = F.stf[0 * nt + it - 1];
+                if (F.comp_mask & 2) sy[j] = F.stf[1 * nt + it - 1];
+                if (F.comp_mask & 4) sz[j] = F.stf[2 * nt + it - 1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < FCH; ++j) {
+            if (mask[j] & 1) fx = fx - w[j] * sx[j];
+            if (mask[j] & 2) fy = fy - w[j] * sy[j];
+            if (mask[j] & 4) fz = fz - w[j] * sz[j];
+        }
     }
     f[2 * p] = fx; f[2 * p + 1] = fy; f[2 * NP + p] = fz;
 }
