@@ -26,3 +26,22 @@ def test_algorithmic_bytes():
     assert abs(a["rho_g"] - (385 * 385 * 193) / (96 * 96 * 48 * 125)) < 1e-12
     # the implemented element kernel moves less than the reference algorithm
     assert bench.implemented_bytes((96, 96, 48), 3, True) < a["element"]
+
+
+def test_small_gradient_workload_builds():
+    """The 8x8x4 gradient workload of the batched-vs-plain legs builds on the
+    host for 32 events: the mesh and model of 8x8x4 elements, one moment
+    source and a receiver list inside the section per event, the first event
+    on the 10x10 surface grid, the others with 10-400 receivers."""
+    import numpy as np
+    from ses3d_amd import configs
+    w = configs.WORKLOADS["gradient_8x8x4"]
+    assert w.nel == (8, 8, 4) and w.n_points == 8 * 8 * 4 * 125
+    run = configs.build(w, seed=0, nt=20, n_events=32)
+    assert (run.nx, run.ny, run.nz) == (7, 7, 3)
+    assert run.model["mu"].size == w.n_points
+    assert len(run.events) == 32
+    counts = [len(e.rec32) for e in run.events]
+    assert counts[0] == 100 and all(10 <= c <= 400 for c in counts[1:])
+    for e in run.events:
+        assert e.nt == 20 and np.all(np.isfinite(e.stf))
