@@ -16,6 +16,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_EXE = os.path.join(HERE, "_ref", "ses3d.exe")
+# the reference objects with SOURCE/ses3d_grad_Q.f90 in place of ses3d_grad.f90,
+# driven by ref_grad_q_harness.f90 (adjoint passes; also writes grad_Q_mu_<rank>)
+REF_GRADQ_EXE = os.path.join(HERE, "_ref", "ses3d_grad_q.exe")
 
 
 class OParams(C.Structure):
@@ -249,7 +252,7 @@ def restore(deg: int, values: np.ndarray, precision: int = 32) -> np.ndarray:
 # ---------------------------------------------------------------------------
 
 def ref_available() -> bool:
-    return os.path.exists(REF_EXE)
+    return os.path.exists(REF_EXE) and os.path.exists(REF_GRADQ_EXE)
 
 
 def build_ref() -> bool:
@@ -262,13 +265,14 @@ def build_ref() -> bool:
     return r.returncode == 0 and ref_available()
 
 
-def run_ref(root: str, timeout: float = 600.0, nprocs: int = 1) -> subprocess.CompletedProcess:
-    """Run the reference binary on a run tree (cwd = root/MAIN)."""
+def run_ref(root: str, timeout: float = 600.0, nprocs: int = 1, exe: str = REF_EXE) -> subprocess.CompletedProcess:
+    """Run the reference binary (or the grad_Q build, exe=REF_GRADQ_EXE) on a
+    run tree (cwd = root/MAIN)."""
     env = dict(os.environ)
     env.setdefault("OMP_NUM_THREADS", "1")
-    cmd = [REF_EXE]
+    cmd = [exe]
     if nprocs > 1:
-        cmd = [shutil.which("mpiexec") or "/opt/conda/bin/mpiexec", "-n", str(nprocs), REF_EXE]
+        cmd = [shutil.which("mpiexec") or "/opt/conda/bin/mpiexec", "-n", str(nprocs), exe]
     def _stack():
         # the reference keeps large automatic arrays on the stack
         # (e.g. SOURCE/ses3d_evolution.f90:19-21)

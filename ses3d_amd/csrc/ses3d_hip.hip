@@ -1180,7 +1180,11 @@ struct StepRunner {
         drop_graphs();
         return 0;
     }
+    // Executable graphs are destroyed only once the stream has drained, so no
+    // pending launch of one outlives it.
     void drop_graphs() {
+        if (graphs.empty()) return;
+        if (stream) (void)hipStreamSynchronize(stream);
         for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
         graphs.clear();
     }
@@ -1571,15 +1575,21 @@ int run_steps(StepRunner *r, int it_begin, int n, int mode, bool record) {
         if (!ge) {
             GraphEntry g;
             g.mode = mode; g.record = record; g.steps = G;
-            hipGraph_t graph;
+            hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeRelaxed));
             int rc = 0;
             for (int k = 0; k < G && !rc; ++k) rc = launch_step(r, mode, record, false);
             hipError_t e = hipStreamEndCapture(r->stream, &graph);
+            // the captured graph is released on every path below
+            struct GraphGuard {
+                hipGraph_t g;
+                ~GraphGuard() {
+                    if (g) (void)hipGraphDestroy(g);
+                }
+            } guard{e == hipSuccess ? graph : nullptr};
             if (rc) return rc;
             if (e != hipSuccess) return fail(SES3D_HIP_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
             HIP_TRY(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
-            HIP_TRY(hipGraphDestroy(graph));
             r->graphs.push_back(g);
             ge = &r->graphs.back();
         }
@@ -1730,6 +1740,15 @@ int ses3d_hip_group::launch_node() {
 namespace {
 
 int refresh_events(ses3d_hip_context *c);
+
+// A checkpointed forward pass belongs to the model, events and stored-field
+// layout it ran with and to the step state it left: anything that changes
+// those (a new model, init, another run of the forward context) retires it,
+// and the adjoint then refuses to start until forward_checkpointed runs again.
+void invalidate_checkpoints(ses3d_hip_context *c) {
+    c->ckpt_ready = false;
+    c->last_seg_ready = false;
+}
 
 // run_steps or autotune_steps, with whole-run and per-kernel timing when profiling
 int timed_run(StepRunner *r, int it_begin, int n, int mode, bool record, bool tune) {
@@ -2157,6 +2176,7 @@ int ses3d_hip_set_model(ses3d_hip_context *c, const float *rhoinv, const float *
         if (A[i] != 0.0f || B[i] != 0.0f || C[i] != 0.0f) c->aniso = true;
     c->model_set = true;
     c->initialised = false;
+    invalidate_checkpoints(c);
     return 0;
 }
 
@@ -2406,6 +2426,7 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     }
     TRY(refresh_events(c));
     c->drop_graphs();
+    invalidate_checkpoints(c);
     c->it_host = -1;
     c->generation += 1;
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -2420,6 +2441,7 @@ static int run_checked(ses3d_hip_context *c, int32_t it_begin, int32_t n_steps, 
         return fail(SES3D_HIP_EINVAL, "step range outside 1..nt");
     if (tune && n_steps < N_TRIALS) return fail(SES3D_HIP_EINVAL, "autotune needs at least 6 steps");
     const int af = c->cfg.adjoint_flag;
+    if (af == 1) invalidate_checkpoints(c);
     if (af == 1 || af == 2) TRY(ensure_full_history(c));
     return timed_run(c, it_begin, n_steps, af, af != 2, tune);
 }
@@ -2677,6 +2699,7 @@ int ses3d_hip_set_history_degree(ses3d_hip_context *c, int32_t deg) {
     }
     c->hdeg = deg;
     c->initialised = false;
+    invalidate_checkpoints(c);
     return 0;
 }
 
@@ -2784,7 +2807,11 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
     if (!a || !f) return fail(SES3D_HIP_EINVAL, "null context");
     if (!a->initialised) return fail(SES3D_HIP_ESTATE, "init the adjoint context first");
     if (a->cfg.adjoint_flag != 2) return fail(SES3D_HIP_ESTATE, "adjoint context needs adjoint_flag=2");
-    if (!f->ckpt_ready) return fail(SES3D_HIP_ESTATE, "run forward_checkpointed on the forward context first");
+    if (!f->initialised || f->cfg.adjoint_flag != 1)
+        return fail(SES3D_HIP_ESTATE, "the forward context must be an initialised adjoint_flag=1 context");
+    if (!f->ckpt_ready)
+        return fail(SES3D_HIP_ESTATE, "no current checkpointed forward: run forward_checkpointed on the forward "
+                                      "context after its last set_model, init or run");
     if (a->cfg.nt != f->cfg.nt || a->NE != f->NE || a->n_events != f->n_events || a->cfg.samp_ad != f->cfg.samp_ad)
         return fail(SES3D_HIP_EINVAL, "forward and adjoint contexts differ in mesh, nt, samp_ad or event count");
     if (a->hdeg != f->hdeg) return fail(SES3D_HIP_EINVAL, "forward and adjoint contexts store fields of different degree");
@@ -2796,9 +2823,11 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
     TRY(ensure_segment_buffer(f, k, f->ckpt_keep_used));
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
         const int s0 = sgi * k + 1, s1 = std::min((sgi + 1) * k, nt);
-        if (sgi >= first_kept && f->last_seg_ready) {
-            // stored by forward_checkpointed; the first replay below overwrites it
-        } else {
+        // a kept trailing segment's steps were stored by forward_checkpointed
+        // (first_kept*k+1..nt in the segment buffer); the first replay below
+        // overwrites them
+        const bool kept = sgi >= first_kept && f->last_seg_ready;
+        if (!kept) {
             // restore the forward state at the start of the segment, replay it into the buffer
             f->last_seg_ready = false;
             for (int ev = 0; ev < f->n_events; ++ev) {
@@ -2815,13 +2844,15 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
             TRY(run_steps(f, s0, s1 - s0 + 1, 1, false));
             HIP_TRY(hipStreamSynchronize(f->stream));
         }
-        // adjoint steps paired with this segment's forward steps
+        // adjoint steps paired with this segment's forward steps, through the
+        // slots of those steps in the segment buffer
         const int a0 = nt + 1 - s1, a1 = nt + 1 - s0;
         for (int ev = 0; ev < a->n_events; ++ev) {
             EventHost &A = a->evh[ev];
             const EventHost &E = f->evh[ev];
+            const std::vector<int> seg_tab = segment_table(E, nt, kept ? first_kept * k + 1 : s0, kept ? nt : s1);
             std::vector<int> tab(nt + 1, -1);
-            for (int it = a0; it <= a1; ++it) tab[it] = E.slot_dev[nt + 1 - it];
+            for (int it = a0; it <= a1; ++it) tab[it] = seg_tab[nt + 1 - it];
             A.hist = E.seg;
             TRY(upload_slots(a, A, tab));
         }
@@ -2930,6 +2961,7 @@ static int group_run_checked(ses3d_hip_group *g, int32_t it_begin, int32_t n_ste
         aniso |= c->aniso ? 1 : 0;
     }
     for (auto *c : g->ctx) {
+        if (g->af == 1) invalidate_checkpoints(c);
         if (g->af == 1 || g->af == 2) TRY(ensure_full_history(c));
         HIP_TRY(hipStreamSynchronize(c->stream));   // member work issued before this run
     }

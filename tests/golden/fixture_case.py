@@ -1,9 +1,11 @@
 """Compact, self-contained SES3D test cases for the golden fixtures.
 
 A fixture stores every input value it needs (geometry, event, receivers, the
-source time function, the moment tensor and a 1D model as per-(k,n) vectors),
-so the run tree can be rebuilt bit for bit on any machine without
-regenerating random numbers or calling transcendental functions.
+source time function, the moment tensor and the model: a 1D model as
+per-(k,n) vectors, or, for the cases in MODEL_3D, whole laterally varying
+float32 arrays in reference layout), so the run tree can be rebuilt bit for bit
+on any machine without regenerating random numbers or calling transcendental
+functions.
 """
 from __future__ import annotations
 
@@ -19,7 +21,17 @@ CASES = {
     "anelastic_adjoint": ((5, 5, 4), 50, 0.2, 1, 10, (49.3, 19.1, 60000.0), 4, True),
     "force_r": ((4, 5, 3), 40, 0.2, 0, 3, (51.0, 21.5, 15000.0), 3, False),
     "aniso_mt": ((6, 5, 4), 40, 0.2, 1, 10, (50.5, 19.7, 40000.0), 3, False),
+    # laterally varying (3D) models; point forces along theta (type 1) and phi
+    # (type 2); an anelastic adjoint whose last step (47) is not a multiple of
+    # samp_ad, with the shear-Q kernel of SOURCE/ses3d_grad_Q.f90
+    "force_theta_3d": ((5, 4, 3), 40, 0.2, 0, 1, (50.6, 19.4, 20000.0), 3, False),
+    "force_phi_3d": ((4, 5, 3), 40, 0.2, 1, 2, (49.5, 20.6, 35000.0), 3, False),
+    "anelastic_3d_adjoint": ((4, 4, 3), 47, 0.2, 1, 10, (50.3, 20.2, 45000.0), 3, True),
 }
+# cases with a laterally varying model: seeded Gaussian-correlated perturbation
+# (200 km correlation, 3% rms) of vp, vs and rho, and of Q (x3) when anelastic
+MODEL_3D = ("force_theta_3d", "force_phi_3d", "anelastic_3d_adjoint")
+MODEL_FIELDS = ("rhoinv", "mu", "lambda", "Q")
 
 
 def compact_inputs(name: str, seed: int = 11) -> dict:
@@ -45,6 +57,14 @@ def compact_inputs(name: str, seed: int = 11) -> dict:
         out["A_z"] = (0.04 * mu * np.cos(z / 9e4)).astype(np.float32)
         out["B_z"] = (-0.03 * mu * np.sin(z / 7e4)).astype(np.float32)
         out["C_z"] = (0.05 * mu * np.cos(z / 5e4)).astype(np.float32)
+    if name in MODEL_3D:
+        pert = M.gaussian_random_field(box, seed=seed + 100, corr_len=200e3, rms=0.03)
+        m3 = M.make_model(box, is_diss=bool(is_diss), perturbation=pert)
+        if is_diss:
+            m3["Q"] = np.asfortranarray((m3["Q"] * (1.0 + 3.0 * pert)).astype(np.float32))
+        for k in MODEL_FIELDS:
+            if k in m3:
+                out[f"model3d_{k}"] = np.asarray(m3[k], np.float32)
     rec = M.surface_receiver_grid(setup, 2, 2)
     out["rec"] = np.stack([rec.colat, rec.lon, rec.depth], 1)
     return out
@@ -70,6 +90,11 @@ def write_tree(fx: dict, root: str, adjoint_flag: int = None) -> None:
              "C": bz(fx["C_z"]) if "C_z" in fx else np.zeros(shape, np.float32, order="F")}
     if is_diss:
         model["Q"] = bz(fx["Q_z"])
+    for k in MODEL_FIELDS:
+        if f"model3d_{k}" in fx:
+            a = np.asarray(fx[f"model3d_{k}"], np.float32)
+            assert a.shape == shape, (k, a.shape, shape)
+            model[k] = np.asfortranarray(a)
     src = [float(v) for v in fx["src"]]
     ev = F.Event(nt=nt, dt=dt, xxs=src[0], yys=src[1], zzs=src[2], source_type=int(fx["source_type"]),
                  moment=tuple(float(v) for v in fx["moment"]), ofd="../DATA/OUTPUT/1/", ssamp=nt,

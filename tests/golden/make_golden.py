@@ -7,10 +7,15 @@ outputs as read back from its own output files:
   seismograms  (3, nr, nt)        from <ofd>/<station>.x/.y/.z
   vx, vy, vz   (nx+1,...,5,5,5)   snapshot files vx_0_<nt> (velocity gathered at step nt)
   grad_*       (nx+1,...,5,5,5)   kernel files (adjoint cases)
+  grad_Q_mu    (nx+1,...,5,5,5)   anelastic adjoint cases: the shear-Q kernel of
+                                  SOURCE/ses3d_grad_Q.f90, written by the grad_Q
+                                  build (oracle/_ref/ses3d_grad_q.exe), which runs
+                                  the adjoint pass of every adjoint case
   history_*    stored forward fields of the first and last stored steps
+               (1D-model adjoint cases)
   saving_vector
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
 
@@ -59,7 +64,7 @@ def make(name: str) -> str:
             sv = F.read_saving_vector(os.path.join(ffd, "saving_vector_0"), nt)
             out["saving_vector"] = sv
             stored = [int(s) for s in sv if s > 0]
-            for tag, step in (("first", stored[0]), ("last", stored[-1])):
+            for tag, step in (() if name in FC.MODEL_3D else (("first", stored[0]), ("last", stored[-1]))):
                 recs = []
                 for fname in ("vx", "vy", "vz", "exx", "eyy", "ezz", "exy", "exz", "eyz"):
                     data = F.read_records(os.path.join(ffd, f"{fname}_0"))
@@ -68,10 +73,11 @@ def make(name: str) -> str:
                 out[f"history_{tag}"] = np.stack(recs)
                 out[f"history_{tag}_step"] = step
             FC.write_adjoint(fx, root, seis)
-            r = PO.run_ref(root)
+            # the grad_Q build: elastic kernels as ses3d_grad.f90 forms them, plus grad_Q_mu
+            r = PO.run_ref(root, exe=PO.REF_GRADQ_EXE)
             if r.returncode != 0:
                 raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
-            for g in ("grad_rho", "grad_cp", "grad_csh", "grad_csv"):
+            for g in ("grad_rho", "grad_cp", "grad_csh", "grad_csv") + (("grad_Q_mu",) if int(fx["is_diss"]) else ()):
                 out[g] = _crop(F.read_field(os.path.join(ofd, f"{g}_0"), FULL), run)
             out["adjoint_vx"] = _crop(F.read_field(os.path.join(ofd, f"vx_0_{nt}"), FULL), run)
         path = os.path.join(HERE, f"{name}.npz")
@@ -84,5 +90,5 @@ def make(name: str) -> str:
 if __name__ == "__main__":
     if not PO.build_ref():
         sys.exit("reference binary unavailable (needs /root/reference and amdflang)")
-    for n in FC.CASES:
+    for n in (sys.argv[1:] or FC.CASES):
         print(make(n))

@@ -264,3 +264,45 @@ def test_anisotropic_adjoint(tmp_path):
         assert rel <= 1e-5, (g, rel)
     f.close()
     a.close()
+
+
+@pytest.mark.parametrize("retire", ["init", "run", "set_model"])
+def test_adjoint_refuses_retired_checkpoints(tmp_path, retire):
+    """A checkpointed forward belongs to the model, events and step state it ran
+    with: after set_model, init or a run of the forward context, the adjoint
+    must refuse (ESTATE) before any kernel runs, instead of replaying stale
+    checkpoints or reading a slot table another run uploaded; a fresh
+    forward_checkpointed restores the exact kernels."""
+    from ses3d_amd.solver import Ses3dError
+    run = make_case(str(tmp_path), nel=(5, 5, 4), nt=40, is_diss=True, adjoint_flag=1, samp_ad=3, seed=21)
+    e = run.events[0]
+    print(f"retire by {retire}", flush=True)
+    f = _ctx(run, 1)
+    f.load_event(0, e)
+    f.init()
+    f.forward_checkpointed(8, keep=2)
+    loc, rows = _adjoint_sources(e, f.seismograms())
+    a = _ctx(run, 2)
+    a.set_adjoint_sources(0, loc, rows)
+    a.init()
+    a.adjoint_checkpointed(f)
+    K1 = a.kernels()
+    if retire == "init":
+        f.init()
+    elif retire == "run":
+        f.run()
+    else:
+        f.set_model(run.model)
+        f.init()
+    a.init()
+    with pytest.raises(Ses3dError, match="error -3"):
+        a.adjoint_checkpointed(f)
+    f.init()
+    f.forward_checkpointed(8, keep=2)
+    a.init()
+    a.adjoint_checkpointed(f)
+    K2 = a.kernels()
+    for g in K1:
+        np.testing.assert_array_equal(K2[g], K1[g], err_msg=g)
+    f.close()
+    a.close()
