@@ -302,6 +302,20 @@ int ses3d_hip_group_get_timing(ses3d_hip_group *grp, double *total_ms, double *e
 int ses3d_hip_project(ses3d_hip_context *ctx, const float *field, int32_t nbx, const float *bx_deg,
                       int32_t nby, const float *by_deg, int32_t nbz, const float *bz_km, float *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): ses3d_hip_project with the same arguments and results, but the
+ * field, block tables and output are staged in a 64 MiB device array the
+ * library holds on each GPU until the process ends (looked up once per device
+ * when a context is created) instead of a buffer allocated per call, so it
+ * calls no hipMalloc or hipFree.  Projections needing more than 64 MiB (the
+ * field's 4 bytes per GLL point, about 16 million points, plus tables and
+ * output) fail with SES3D_HIP_EINVAL.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other host threads, other contexts
+ * on the same device) race and must be serialised by the caller. */
+int ses3d_hip_project_noalloc(ses3d_hip_context *ctx, const float *field, int32_t nbx, const float *bx_deg,
+                              int32_t nby, const float *by_deg, int32_t nbz, const float *bz_km, float *out);
+
 #ifdef __cplusplus
 }
 #endif

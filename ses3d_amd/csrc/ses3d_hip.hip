@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2065,6 +2066,106 @@ int ensure_checkpoints(ses3d_hip_context *c, int k, int keep) {
 }  // namespace
 
 // ============================================================================
+// scratch of the no-allocation projection entry point
+// ============================================================================
+// Device memory of ses3d_hip_project_noalloc: the staged field, the block
+// tables and the staged output of one projection.  Each GPU the library runs
+// on holds its own copy until the process ends; calls on one GPU share it.
+#define SES3D_NOALLOC_SCRATCH_BYTES (64LL << 20)
+static __device__ __align__(256) unsigned char scratch[SES3D_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;   // device address of `scratch`, per device
+
+// Looked up once per device, with that device current (ses3d_hip_create);
+// host code uses only these addresses, never the array's name.
+int scratch_lookup(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) return fail(SES3D_HIP_EDEVICE, "invalid device index");
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (g_scratch_ptr[(size_t)device]) return 0;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SES3D_HIP_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[(size_t)device] = p;
+    return 0;
+}
+
+void *scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && (size_t)device < g_scratch_ptr.size() ? g_scratch_ptr[(size_t)device] : nullptr;
+}
+
+// A projection's block tables and its device buffer's extent: the field
+// (reference layout), the output, then per axis the block starts, element
+// indices and integrals.
+struct ProjPlan {
+    AxisTerms T[3];
+    long long nb = 0;
+    float vol = 0;
+    size_t bytes = 0;
+};
+
+int plan_projection(const ses3d_hip_context *c, int32_t nbx, const float *bx_deg, int32_t nby, const float *by_deg,
+                    int32_t nbz, const float *bz_km, ProjPlan &P) {
+    if (nbx < 2 || nby < 2 || nbz < 2) return fail(SES3D_HIP_EINVAL, "a block grid needs at least 2 edges per axis");
+    const ses3d::MeshSetup &ms = c->ms;
+    // edges as project_kernel.f90:381-388 converts them (pi of SOURCE/ses3d_modules.f90:25)
+    const float pi = 3.1415926535898f;
+    std::vector<float> ex(nbx), ey(nby), ez(nbz);
+    for (int i = 0; i < nbx; ++i) ex[i] = (bx_deg[i] * pi) / 180.0f;
+    for (int i = 0; i < nby; ++i) ey[i] = (by_deg[i] * pi) / 180.0f;
+    for (int i = 0; i < nbz; ++i) ez[i] = bz_km[i] * 1000.0f;
+    P.vol = ((bx_deg[1] - bx_deg[0]) * (by_deg[1] - by_deg[0])) * (bz_km[1] - bz_km[0]);
+    P.T[0] = axis_terms(ms.x, ms.nx, ms.xmin, ms.xmax, ms.dx, ms.knots, ex, false);
+    P.T[1] = axis_terms(ms.y, ms.ny, ms.ymin, ms.ymax, ms.dy, ms.knots, ey, false);
+    P.T[2] = axis_terms(ms.z, ms.nz, ms.zmin, ms.zmax, ms.dz, ms.knots, ez, true);
+    P.nb = (long long)(nbx - 1) * (nby - 1) * (nbz - 1);
+    if (P.nb >= (1LL << 31)) return fail(SES3D_HIP_EINVAL, "block grid too large");
+    P.bytes = sizeof(float) * ((size_t)c->NPTS + (size_t)P.nb);
+    for (const auto &t : P.T) P.bytes += sizeof(int) * (t.start.size() + t.elem.size() + 1) + sizeof(float) * (t.w.size() + 1);
+    return 0;
+}
+
+// Stage the field and the tables in `buf` (P.bytes of device memory), run
+// k_project on the context's stream and copy the block values to `out`.
+int run_projection(ses3d_hip_context *c, const ProjPlan &P, const float *field, int32_t nbx, int32_t nby,
+                   int32_t nbz, float *out, char *buf) {
+    int rc = 0;
+    float *u_d = (float *)buf;
+    float *out_d = u_d + c->NPTS;
+    int *ints = (int *)(out_d + P.nb);
+    ProjAxis PA[3];
+    for (int a = 0; a < 3; ++a) {
+        const AxisTerms &t = P.T[a];
+        int *st = ints;
+        int *el = st + t.start.size();
+        float *w = (float *)(el + t.elem.size() + 1);
+        ints = (int *)(w + t.w.size() + 1);
+        PA[a] = {st, el, w};
+        if (!rc && hipMemcpyAsync(st, t.start.data(), sizeof(int) * t.start.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
+        if (!rc && !t.elem.empty() && hipMemcpyAsync(el, t.elem.data(), sizeof(int) * t.elem.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
+        if (!rc && !t.w.empty() && hipMemcpyAsync(w, t.w.data(), sizeof(float) * t.w.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
+    }
+    if (!rc && hipMemcpyAsync(u_d, field, sizeof(float) * c->NPTS, hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
+    if (!rc) {
+        hipLaunchKernelGGL(k_project, dim3((unsigned)((P.nb + 127) / 128)), dim3(128), 0, c->stream, u_d, c->NX, c->NY,
+                           c->NE, PA[0], PA[1], PA[2], nbx - 1, nby - 1, nbz - 1, P.vol, out_d);
+        if (hipGetLastError() != hipSuccess) rc = 1;
+    }
+    if (!rc && hipMemcpyAsync(out, out_d, sizeof(float) * P.nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess) rc = 1;
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = 1;
+    (void)hipStreamSynchronize(c->stream);
+    if (rc) return fail(SES3D_HIP_EDEVICE, "kernel projection failed on the device");
+    return 0;
+}
+
+}  // namespace
+
+// ============================================================================
 // C-ABI
 // ============================================================================
 extern "C" {
@@ -2107,6 +2208,14 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(SES3D_HIP_EDEVICE, "hipStreamCreate failed");
+    }
+    {
+        const int rc = scratch_lookup(c->device);   // the device is current here
+        if (rc) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return rc;
+        }
     }
     c->evh.resize(n_events);
     c->ms.build(cfg->nx, cfg->ny, cfg->nz, cfg->pml, cfg->xmin, cfg->xmax, cfg->ymin, cfg->ymax, cfg->zmin,
@@ -3049,55 +3158,29 @@ int ses3d_hip_group_get_timing(ses3d_hip_group *g, double *total_ms, double *ele
 int ses3d_hip_project(ses3d_hip_context *c, const float *field, int32_t nbx, const float *bx_deg, int32_t nby,
                       const float *by_deg, int32_t nbz, const float *bz_km, float *out) {
     if (!c || !field || !bx_deg || !by_deg || !bz_km || !out) return fail(SES3D_HIP_EINVAL, "null argument");
-    if (nbx < 2 || nby < 2 || nbz < 2) return fail(SES3D_HIP_EINVAL, "a block grid needs at least 2 edges per axis");
-    const ses3d::MeshSetup &ms = c->ms;
-    // edges as project_kernel.f90:381-388 converts them (pi of SOURCE/ses3d_modules.f90:25)
-    const float pi = 3.1415926535898f;
-    std::vector<float> ex(nbx), ey(nby), ez(nbz);
-    for (int i = 0; i < nbx; ++i) ex[i] = (bx_deg[i] * pi) / 180.0f;
-    for (int i = 0; i < nby; ++i) ey[i] = (by_deg[i] * pi) / 180.0f;
-    for (int i = 0; i < nbz; ++i) ez[i] = bz_km[i] * 1000.0f;
-    const float vol = ((bx_deg[1] - bx_deg[0]) * (by_deg[1] - by_deg[0])) * (bz_km[1] - bz_km[0]);
-    const AxisTerms TX = axis_terms(ms.x, ms.nx, ms.xmin, ms.xmax, ms.dx, ms.knots, ex, false);
-    const AxisTerms TY = axis_terms(ms.y, ms.ny, ms.ymin, ms.ymax, ms.dy, ms.knots, ey, false);
-    const AxisTerms TZ = axis_terms(ms.z, ms.nz, ms.zmin, ms.zmax, ms.dz, ms.knots, ez, true);
-    const long long nb = (long long)(nbx - 1) * (nby - 1) * (nbz - 1);
-    if (nb >= (1LL << 31)) return fail(SES3D_HIP_EINVAL, "block grid too large");
+    ProjPlan P;
+    TRY(plan_projection(c, nbx, bx_deg, nby, by_deg, nbz, bz_km, P));
     // one device buffer for the field, the tables and the output (freed before return)
-    const AxisTerms *T[3] = {&TX, &TY, &TZ};
-    size_t bytes = sizeof(float) * ((size_t)c->NPTS + (size_t)nb);
-    for (auto *t : T) bytes += sizeof(int) * (t->start.size() + t->elem.size() + 1) + sizeof(float) * (t->w.size() + 1);
     char *buf = nullptr;
-    HIP_TRY(hipMalloc((void **)&buf, bytes));
-    int rc = 0;
-    // field, output, then per axis: block starts, element indices, integrals
-    float *u_d = (float *)buf;
-    float *out_d = u_d + c->NPTS;
-    int *ints = (int *)(out_d + nb);
-    ProjAxis PA[3];
-    for (int a = 0; a < 3; ++a) {
-        const AxisTerms &t = *T[a];
-        int *st = ints;
-        int *el = st + t.start.size();
-        float *w = (float *)(el + t.elem.size() + 1);
-        ints = (int *)(w + t.w.size() + 1);
-        PA[a] = {st, el, w};
-        if (!rc && hipMemcpyAsync(st, t.start.data(), sizeof(int) * t.start.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
-        if (!rc && !t.elem.empty() && hipMemcpyAsync(el, t.elem.data(), sizeof(int) * t.elem.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
-        if (!rc && !t.w.empty() && hipMemcpyAsync(w, t.w.data(), sizeof(float) * t.w.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
-    }
-    if (!rc && hipMemcpyAsync(u_d, field, sizeof(float) * c->NPTS, hipMemcpyHostToDevice, c->stream) != hipSuccess) rc = 1;
-    if (!rc) {
-        hipLaunchKernelGGL(k_project, dim3((unsigned)((nb + 127) / 128)), dim3(128), 0, c->stream, u_d, c->NX, c->NY,
-                           c->NE, PA[0], PA[1], PA[2], nbx - 1, nby - 1, nbz - 1, vol, out_d);
-        if (hipGetLastError() != hipSuccess) rc = 1;
-    }
-    if (!rc && hipMemcpyAsync(out, out_d, sizeof(float) * nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess) rc = 1;
-    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = 1;
-    (void)hipStreamSynchronize(c->stream);
+    HIP_TRY(hipMalloc((void **)&buf, P.bytes));
+    const int rc = run_projection(c, P, field, nbx, nby, nbz, out, buf);
     (void)hipFree(buf);
-    if (rc) return fail(SES3D_HIP_EDEVICE, "kernel projection failed on the device");
-    return 0;
+    return rc;
+}
+
+int ses3d_hip_project_noalloc(ses3d_hip_context *c, const float *field, int32_t nbx, const float *bx_deg,
+                              int32_t nby, const float *by_deg, int32_t nbz, const float *bz_km, float *out) {
+    if (!c || !field || !bx_deg || !by_deg || !bz_km || !out) return fail(SES3D_HIP_EINVAL, "null argument");
+    ProjPlan P;
+    TRY(plan_projection(c, nbx, bx_deg, nby, by_deg, nbz, bz_km, P));
+    if (P.bytes > (size_t)SES3D_NOALLOC_SCRATCH_BYTES)
+        return fail(SES3D_HIP_EINVAL, "projection needs " + std::to_string(P.bytes) +
+                                          " bytes of scratch (field + block tables + output); the no-allocation "
+                                          "entry point holds at most " +
+                                          std::to_string((long long)SES3D_NOALLOC_SCRATCH_BYTES) + " bytes");
+    char *buf = (char *)scratch_address(c->device);
+    if (!buf) return fail(SES3D_HIP_ESTATE, "no-allocation scratch was not set up on this context's device");
+    return run_projection(c, P, field, nbx, nby, nbz, out, buf);
 }
 
 }  // extern "C"

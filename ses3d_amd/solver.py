@@ -105,6 +105,7 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_group_set_profiling": ([P, i32], C.c_int),
         "ses3d_hip_group_get_timing": ([P, dp, dp, dp, ip], C.c_int),
         "ses3d_hip_project": ([P, fp, i32, fp, i32, fp, i32, fp, fp], C.c_int),
+        "ses3d_hip_project_noalloc": ([P, fp, i32, fp, i32, fp, i32, fp, fp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -358,15 +359,18 @@ class Solver:
         _check(self.lib.ses3d_hip_adjoint_checkpointed(self.h, fwd.h))
         self.synchronize()
 
-    def project(self, field, bx_deg, by_deg, bz_km) -> np.ndarray:
+    def project(self, field, bx_deg, by_deg, bz_km, no_alloc: bool = False) -> np.ndarray:
         """Block-grid projection of a reference-layout field as
         TOOLS/SOURCE/project_kernel.f90 projects a kernel (one subvolume);
-        returns (nbx-1, nby-1, nbz-1) float32."""
+        returns (nbx-1, nby-1, nbz-1) float32.  no_alloc stages the data in
+        the library's fixed per-GPU scratch instead of a per-call device
+        buffer (ses3d_hip_project_noalloc; at most 64 MiB)."""
         f = _ref_f32(field)
         e = [np.ascontiguousarray(a, np.float32) for a in (bx_deg, by_deg, bz_km)]
         out = np.zeros((len(e[0]) - 1) * (len(e[1]) - 1) * (len(e[2]) - 1), np.float32)
-        _check(self.lib.ses3d_hip_project(self.h, _fptr(f), len(e[0]), _fptr(e[0]), len(e[1]), _fptr(e[1]),
-                                          len(e[2]), _fptr(e[2]), _fptr(out)))
+        fn = self.lib.ses3d_hip_project_noalloc if no_alloc else self.lib.ses3d_hip_project
+        _check(fn(self.h, _fptr(f), len(e[0]), _fptr(e[0]), len(e[1]), _fptr(e[1]), len(e[2]), _fptr(e[2]),
+                  _fptr(out)))
         return out.reshape(len(e[0]) - 1, len(e[1]) - 1, len(e[2]) - 1)
 
     def project_kernels(self, bx_deg, by_deg, bz_km, ev: int = 0) -> dict:
