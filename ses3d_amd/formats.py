@@ -81,8 +81,11 @@ class _Lines:
                 return _tok(line)
 
     def skip(self, n: int = 1) -> None:
+        """n header reads ``read(u,*) junk`` of a character item: list-directed
+        input passes over blank records, so each consumes the next non-blank
+        record (a blank first line of an ad_src file is not a header line)."""
         for _ in range(n):
-            self.next()
+            self.value()
 
 
 # ---------------------------------------------------------------------------
@@ -266,7 +269,9 @@ def write_stf(path: str, samples: np.ndarray, dt: float = 0.0, note: str = "") -
 
 
 def read_stf(path: str, nt: int) -> np.ndarray:
-    """Skip 4 header lines, then nt samples (SOURCE/ses3d_input.f90:419-428)."""
+    """4 header reads, then nt samples (SOURCE/ses3d_input.f90:419-428).  A file
+    without the header (as INPUT/stf of the reference is) loses its first 4
+    samples, as in the reference."""
     L = _Lines(path)
     L.skip(4)
     return np.array([f32(L.value()) for _ in range(nt)], np.float32)
@@ -592,7 +597,8 @@ def write_ad_src(path: str, loc_deg, adsrc_xyz_forward_time: np.ndarray) -> None
 
 
 def read_ad_src(path: str, nt: int) -> np.ndarray:
-    """-> (nt,3) float32 rows as stored (already reversed), SOURCE/ses3d_init.f90:479-490."""
+    """-> (nt,3) float32 rows as stored (already reversed), SOURCE/ses3d_init.f90:479-490:
+    4 header reads (blank records skipped), then nt rows."""
     L = _Lines(path)
     L.skip(4)
     out = np.zeros((nt, 3), np.float32)
