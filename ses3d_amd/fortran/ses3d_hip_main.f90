@@ -4,15 +4,22 @@
 ! Launched from MAIN/ like the reference (MAIN/job_run_ses3d); reads the same
 ! files with the same relative paths and writes the same outputs:
 !   ../INPUT/event_list, setup, event_<ev>, relax, stf, recfile_<ev>
-!   ../MODELS/MODELS/boxfile and rhoinv0 mu0 lambda0 A0 B0 C0 [Q0]
-!   ../ADJOINT/<ev>/ad_srcfile, ad_src_<k>          (adjoint_flag=2)
-!   <ofd>/<station>.x/.y/.z, vx_0_<it> ...          (forward)
-!   <ffd>/<ev>/vx_0 ... eyz_0, saving_vector_0      (adjoint_flag=1)
-!   <ofd>/grad_rho_0, grad_cp_0, grad_csh_0, grad_csv_0 (adjoint_flag=2)
+!   ../MODELS/MODELS/boxfile and rhoinv<r> mu<r> lambda<r> A<r> B<r> C<r> [Q<r>]
+!   ../ADJOINT/<ev>/ad_srcfile, ad_src_<k>              (adjoint_flag=2)
+!   <ofd>/<station>.x/.y/.z, vx_<r>_<it> ...            (forward)
+!   <ffd>/<ev>/vx_<r> ... eyz_<r>, saving_vector_<r>    (adjoint_flag=1)
+!   <ofd>/grad_rho_<r>, grad_cp_<r>, grad_csh_<r>, grad_csv_<r> (adjoint_flag=2)
+! for every rank box r of the boxfile (px*py*pz of them, as the reference's
+! MPI ranks would read and write them): the boxes' model records are placed
+! into one section (SOURCE/ses3d_input.f90:49-170 gives each rank its box,
+! :472-501 its records), the section is solved on the GPU, and each box's part
+! of the outputs goes to that box's files, in the box's own extent.
+! The absorbing-layer width is the reference's compile-time pml
+! (SOURCE/ses3d_modules.f90:21, default 2); here it is read at run time from
+! the command line (ses3d_hip.exe --pml N) or SES3D_PML.
 ! Structure follows SOURCE/ses3d_main.f90:77-182 with ses3d_input
-! (SOURCE/ses3d_input.f90) restated for one box; the time step, seismograms,
+! (SOURCE/ses3d_input.f90) restated; the time step, seismograms,
 ! forward-field storage and kernels run in libses3d_hip.so.
-! The boxfile must describe a single box: the whole section lives on one GPU.
 !==============================================================================
 program ses3d_hip_main
   use iso_c_binding
@@ -25,9 +32,12 @@ program ses3d_hip_main
   character(len=1000) :: junk, ofd, ffd, line
   character(len=1000), allocatable :: event_indices(:)
   character(len=12), allocatable :: station_name(:)
-  character(len=40) :: s_idx, s_it
-  integer :: n_events, i_events, pp, px, py, pz, k, i, it, ios
-  integer :: ix_multi, iy_multi, iz_multi, nxmin, nxmax, nymin, nymax, nzmin, nzmax
+  character(len=40) :: s_idx, s_it, s_rank
+  integer :: n_events, i_events, pp, px, py, pz, k, i, it, ios, ib, pml
+  ! rank boxes: multi index, element index ranges, physical bounds; offsets and
+  ! extents of each box's elements in the section
+  integer, allocatable :: bm(:,:), blo(:,:), bhi(:,:), boff(:,:), bn(:,:)
+  real, allocatable :: bmin(:,:), bmax(:,:)
   integer :: nt, source_type, is_diss, adjoint_flag, samp_ad, ssamp, output_displacement
   integer :: nr_global, nr, nad, npts, n_stored, nsnap
   real :: xmin, xmax, ymin, ymax, zmin, zmax, dt
@@ -43,6 +53,7 @@ program ses3d_hip_main
   type(c_ptr) :: ctx
   character(len=3), parameter :: hname(9) = ['vx ', 'vy ', 'vz ', 'exx', 'eyy', 'ezz', 'exy', 'exz', 'eyz']
 
+  pml = absorbing_width()
   call system('mkdir -p ../DATA/LOGFILES')
   open(unit=98, file='../INPUT/event_list', status='old', action='read')
   read(98, *) n_events
@@ -56,6 +67,7 @@ program ses3d_hip_main
 
   do i_events = 1, n_events
      !---------------------------------------------------------- boxfile
+     ! SOURCE/ses3d_input.f90:49-99: every rank box of the section
      open(unit=15, file='../MODELS/MODELS/boxfile', status='old', action='read')
      do k = 1, 14
         read(15, *) junk
@@ -64,21 +76,29 @@ program ses3d_hip_main
      read(15, *) px
      read(15, *) py
      read(15, *) pz
-     if (pp /= 1) then
-        write(*, *) 'the boxfile describes ', pp, ' boxes; one GPU runs the whole section as one box'
+     read(15, *) junk
+     if (pp /= px * py * pz .or. pp < 1) then
+        write(*, *) 'boxfile: ', pp, ' boxes for ', px, 'x', py, 'x', pz, ' processes'
         stop 1
      end if
-     read(15, *) junk
-     read(15, *) k
-     read(15, *) ix_multi, iy_multi, iz_multi
-     read(15, *) nxmin, nxmax
-     read(15, *) nymin, nymax
-     read(15, *) nzmin, nzmax
-     read(15, *) xmin, xmax
-     read(15, *) ymin, ymax
-     read(15, *) zmin, zmax
+     allocate(bm(3, pp), blo(3, pp), bhi(3, pp), boff(3, pp), bn(3, pp), bmin(3, pp), bmax(3, pp))
+     do ib = 1, pp
+        read(15, *) k
+        read(15, *) bm(1, ib), bm(2, ib), bm(3, ib)
+        read(15, *) blo(1, ib), bhi(1, ib)
+        read(15, *) blo(2, ib), bhi(2, ib)
+        read(15, *) blo(3, ib), bhi(3, ib)
+        read(15, *) bmin(1, ib), bmax(1, ib)
+        read(15, *) bmin(2, ib), bmax(2, ib)
+        read(15, *) bmin(3, ib), bmax(3, ib)
+        read(15, *) junk
+        if (k /= ib) then
+           write(*, *) 'boxfile: box ', ib, ' is numbered ', k
+           stop 1
+        end if
+     end do
      close(15)
-     nx = nxmax - nxmin; ny = nymax - nymin; nz = nzmax - nzmin
+     call place_boxes()
      npts = (nx + 1) * (ny + 1) * (nz + 1) * (lpd + 1)**3
      !---------------------------------------------------------- event file
      open(unit=15, file='../INPUT/event_'//trim(event_indices(i_events)), status='old', action='read')
@@ -137,16 +157,18 @@ program ses3d_hip_main
      read(15, *) D_p(3)
      close(15)
      !---------------------------------------------------------- model
+     ! each box's records in its part of the section (SOURCE/ses3d_input.f90:472-501)
      allocate(rhoinv(npts), mu(npts), lambda(npts), A(npts), B(npts), C(npts), Q(npts))
-     call read_field('../MODELS/MODELS/rhoinv0', rhoinv)
-     call read_field('../MODELS/MODELS/mu0', mu)
-     call read_field('../MODELS/MODELS/lambda0', lambda)
-     call read_field('../MODELS/MODELS/A0', A)
-     call read_field('../MODELS/MODELS/B0', B)
-     call read_field('../MODELS/MODELS/C0', C)
-     if (is_diss == 1) call read_field('../MODELS/MODELS/Q0', Q)
+     Q = 1.0
+     call read_boxes('../MODELS/MODELS/rhoinv', rhoinv)
+     call read_boxes('../MODELS/MODELS/mu', mu)
+     call read_boxes('../MODELS/MODELS/lambda', lambda)
+     call read_boxes('../MODELS/MODELS/A', A)
+     call read_boxes('../MODELS/MODELS/B', B)
+     call read_boxes('../MODELS/MODELS/C', C)
+     if (is_diss == 1) call read_boxes('../MODELS/MODELS/Q', Q)
      !---------------------------------------------------------- context
-     cfg%nx = nx; cfg%ny = ny; cfg%nz = nz; cfg%lpd = lpd; cfg%pml = 2
+     cfg%nx = nx; cfg%ny = ny; cfg%nz = nz; cfg%lpd = lpd; cfg%pml = pml
      cfg%is_diss = is_diss; cfg%adjoint_flag = adjoint_flag; cfg%samp_ad = samp_ad; cfg%nt = nt
      cfg%dt = dt; cfg%xmin = xmin; cfg%xmax = xmax; cfg%ymin = ymin; cfg%ymax = ymax
      cfg%zmin = zmin; cfg%zmax = zmax; cfg%zmax_global = zmax_g
@@ -222,22 +244,32 @@ program ses3d_hip_main
         end do
         close(10)
         n_stored = count(sv > 0)
-        allocate(hist(9 * npts), field(npts))
-        do k = 1, 9
-           open(unit=20 + k, file=trim(ffd)//trim(hname(k))//'_0', action='read', form='unformatted')
+        allocate(hist(9 * npts))
+        do ib = 1, pp
+           write(s_rank, '(I0)') ib - 1
+           do k = 1, 9
+              open(unit=100 + 9 * (ib - 1) + k, file=trim(ffd)//trim(hname(k))//'_'//trim(s_rank), &
+                   action='read', form='unformatted')
+           end do
         end do
         do it = 1, nt
            if (sv(it) <= 0) cycle
-           do k = 1, 9
-              read(20 + k) field
-              hist((k - 1) * npts + 1:k * npts) = field
+           do ib = 1, pp
+              allocate(field(box_points(ib)))
+              do k = 1, 9
+                 read(100 + 9 * (ib - 1) + k) field
+                 call place(hist((k - 1) * npts + 1:k * npts), field, ib)
+              end do
+              deallocate(field)
            end do
            call hip_check(ses3d_hip_put_history(ctx, 0_c_int32_t, int(it, c_int32_t), hist), 'put_history')
         end do
-        do k = 1, 9
-           close(20 + k)
+        do ib = 1, pp
+           do k = 1, 9
+              close(100 + 9 * (ib - 1) + k)
+           end do
         end do
-        deallocate(hist, field, sv)
+        deallocate(hist, sv)
      end if
 
      !---------------------------------------------------------- time loop
@@ -262,9 +294,9 @@ program ses3d_hip_main
            call hip_check(ses3d_hip_get_field(ctx, 0_c_int32_t, 'vz'//c_null_char, vzs), 'get_field')
            call hip_check(ses3d_hip_run(ctx, int(nsnap, c_int32_t), 1_c_int32_t), 'run')
            write(s_it, '(I0)') nsnap
-           call write_field(trim(ofd)//'vx_0_'//trim(s_it), vxs)
-           call write_field(trim(ofd)//'vy_0_'//trim(s_it), vys)
-           call write_field(trim(ofd)//'vz_0_'//trim(s_it), vzs)
+           call write_boxes(trim(ofd)//'vx_', '_'//trim(s_it), vxs)
+           call write_boxes(trim(ofd)//'vy_', '_'//trim(s_it), vys)
+           call write_boxes(trim(ofd)//'vz_', '_'//trim(s_it), vzs)
            it = nsnap + 1
         else
            call hip_check(ses3d_hip_run(ctx, int(it, c_int32_t), int(nt - it + 1, c_int32_t)), 'run')
@@ -280,34 +312,46 @@ program ses3d_hip_main
         ! forward field records and saving vector (SOURCE/ses3d_output.f90:138-156, :379-392)
         allocate(sv(nt), hist(9 * npts))
         call hip_check(ses3d_hip_get_saving_vector(ctx, sv), 'get_saving_vector')
-        do k = 1, 9
-           open(unit=20 + k, file=trim(ffd)//trim(hname(k))//'_0', action='write', form='unformatted')
+        do ib = 1, pp
+           write(s_rank, '(I0)') ib - 1
+           do k = 1, 9
+              open(unit=100 + 9 * (ib - 1) + k, file=trim(ffd)//trim(hname(k))//'_'//trim(s_rank), &
+                   action='write', form='unformatted')
+           end do
         end do
         do it = 1, nt
            if (sv(it) <= 0) cycle
            call hip_check(ses3d_hip_get_history(ctx, 0_c_int32_t, int(it, c_int32_t), hist), 'get_history')
-           do k = 1, 9
-              write(20 + k) hist((k - 1) * npts + 1:k * npts)
+           do ib = 1, pp
+              allocate(field(box_points(ib)))
+              do k = 1, 9
+                 call take(field, hist((k - 1) * npts + 1:k * npts), ib)
+                 write(100 + 9 * (ib - 1) + k) field
+              end do
+              deallocate(field)
            end do
         end do
-        do k = 1, 9
-           close(20 + k)
+        do ib = 1, pp
+           do k = 1, 9
+              close(100 + 9 * (ib - 1) + k)
+           end do
+           write(s_rank, '(I0)') ib - 1
+           open(unit=10, file=trim(ffd)//'saving_vector_'//trim(s_rank), action='write')
+           do k = 1, nt
+              write(10, *) sv(k)
+           end do
+           close(10)
         end do
-        open(unit=10, file=trim(ffd)//'saving_vector_0', action='write')
-        do k = 1, nt
-           write(10, *) sv(k)
-        end do
-        close(10)
         deallocate(sv, hist)
      end if
      if (adjoint_flag == 2) then
         ! kernels (SOURCE/ses3d_output.f90:318-343): float32 sums, returned widened, written as float32
         allocate(g1(npts), g2(npts), g3(npts), g4(npts), field(npts))
         call hip_check(ses3d_hip_get_kernels(ctx, 0_c_int32_t, g1, g2, g3, g4, c_null_ptr), 'get_kernels')
-        field = real(g1); call write_field(trim(ofd)//'grad_rho_0', field)
-        field = real(g2); call write_field(trim(ofd)//'grad_cp_0', field)
-        field = real(g3); call write_field(trim(ofd)//'grad_csh_0', field)
-        field = real(g4); call write_field(trim(ofd)//'grad_csv_0', field)
+        field = real(g1); call write_boxes(trim(ofd)//'grad_rho_', '', field)
+        field = real(g2); call write_boxes(trim(ofd)//'grad_cp_', '', field)
+        field = real(g3); call write_boxes(trim(ofd)//'grad_csh_', '', field)
+        field = real(g4); call write_boxes(trim(ofd)//'grad_csv_', '', field)
         deallocate(g1, g2, g3, g4, field)
         deallocate(adloc, adstf)
      else
@@ -319,10 +363,147 @@ program ses3d_hip_main
         deallocate(seis, ridx, so, station_name, rec)
      end if
      call ses3d_hip_destroy(ctx)
+     deallocate(bm, blo, bhi, boff, bn, bmin, bmax)
   end do
   close(99)
 
 contains
+
+  ! Absorbing-layer width in elements: --pml N on the command line, else
+  ! SES3D_PML, else the reference's default of 2.
+  integer function absorbing_width()
+    character(len=64) :: arg
+    integer :: n, st
+    absorbing_width = 2
+    call get_environment_variable('SES3D_PML', arg, status=st)
+    if (st == 0 .and. len_trim(arg) > 0) read(arg, *) absorbing_width
+    do n = 1, command_argument_count() - 1
+       call get_command_argument(n, arg)
+       if (trim(arg) == '--pml') then
+          call get_command_argument(n + 1, arg)
+          read(arg, *) absorbing_width
+       end if
+    end do
+  end function absorbing_width
+
+  ! The section the boxes tile.  Box b (multi index (i,j,k)) holds elements
+  ! nx_min+i-1 .. nx_max+i-1 along theta, likewise along phi, counted from the
+  ! section's minimum; radially nz_min+k-1 .. nz_max+k-1 counted from zmin,
+  ! while element index k of the arrays runs down from zmax
+  ! (MODELS/SOURCE/generate_models.f90:200-282, SOURCE/ses3d_init.f90:239).
+  subroutine place_boxes()
+    integer :: a, ntot(3), nsum
+    do a = 1, 3
+       ntot(a) = maxval(bhi(a, :) + bm(a, :))
+    end do
+    nsum = 0
+    do ib = 1, pp
+       do a = 1, 3
+          bn(a, ib) = bhi(a, ib) - blo(a, ib) + 1
+          boff(a, ib) = blo(a, ib) + bm(a, ib) - 1
+       end do
+       boff(3, ib) = ntot(3) - (bhi(3, ib) + bm(3, ib))
+       nsum = nsum + bn(1, ib) * bn(2, ib) * bn(3, ib)
+       if (any(boff(:, ib) < 0) .or. any(boff(:, ib) + bn(:, ib) > ntot)) then
+          write(*, *) 'boxfile: box ', ib, ' lies outside the section'
+          stop 1
+       end if
+    end do
+    if (nsum /= ntot(1) * ntot(2) * ntot(3)) then
+       write(*, *) 'boxfile: the boxes do not tile a section of ', ntot(1), 'x', ntot(2), 'x', ntot(3), ' elements'
+       stop 1
+    end if
+    nx = ntot(1) - 1; ny = ntot(2) - 1; nz = ntot(3) - 1
+    do ib = 1, pp
+       if (bm(1, ib) == 1) xmin = bmin(1, ib)
+       if (bm(1, ib) == px) xmax = bmax(1, ib)
+       if (bm(2, ib) == 1) ymin = bmin(2, ib)
+       if (bm(2, ib) == py) ymax = bmax(2, ib)
+       if (bm(3, ib) == 1) zmin = bmin(3, ib)
+       if (bm(3, ib) == pz) zmax = bmax(3, ib)
+    end do
+  end subroutine place_boxes
+
+  integer function box_points(bb)
+    integer, intent(in) :: bb
+    box_points = bn(1, bb) * bn(2, bb) * bn(3, bb) * (lpd + 1)**3
+  end function box_points
+
+  ! copy box bb's field (its own extent, reference layout) into the section's
+  subroutine place(g, a, bb)
+    real, intent(inout) :: g(:)
+    real, intent(in) :: a(:)
+    integer, intent(in) :: bb
+    integer :: ii, jj, kk, q
+    integer(8) :: ia, ig
+    ia = 0
+    do q = 0, (lpd + 1)**3 - 1
+       do kk = 0, bn(3, bb) - 1
+          do jj = 0, bn(2, bb) - 1
+             do ii = 0, bn(1, bb) - 1
+                ia = ia + 1
+                ig = 1 + (ii + boff(1, bb)) + int(nx + 1, 8) * ((jj + boff(2, bb)) + int(ny + 1, 8) * &
+                     ((kk + boff(3, bb)) + int(nz + 1, 8) * q))
+                g(ig) = a(ia)
+             end do
+          end do
+       end do
+    end do
+  end subroutine place
+
+  ! box bb's part of a section field
+  subroutine take(a, g, bb)
+    real, intent(out) :: a(:)
+    real, intent(in) :: g(:)
+    integer, intent(in) :: bb
+    integer :: ii, jj, kk, q
+    integer(8) :: ia, ig
+    ia = 0
+    do q = 0, (lpd + 1)**3 - 1
+       do kk = 0, bn(3, bb) - 1
+          do jj = 0, bn(2, bb) - 1
+             do ii = 0, bn(1, bb) - 1
+                ia = ia + 1
+                ig = 1 + (ii + boff(1, bb)) + int(nx + 1, 8) * ((jj + boff(2, bb)) + int(ny + 1, 8) * &
+                     ((kk + boff(3, bb)) + int(nz + 1, 8) * q))
+                a(ia) = g(ig)
+             end do
+          end do
+       end do
+    end do
+  end subroutine take
+
+  ! <prefix><rank> of every box into the section field
+  subroutine read_boxes(prefix, g)
+    character(len=*), intent(in) :: prefix
+    real, intent(inout) :: g(:)
+    real, allocatable :: a(:)
+    character(len=40) :: sr
+    integer :: bb
+    do bb = 1, pp
+       write(sr, '(I0)') bb - 1
+       allocate(a(box_points(bb)))
+       call read_field(prefix//trim(sr), a)
+       call place(g, a, bb)
+       deallocate(a)
+    end do
+  end subroutine read_boxes
+
+  ! each box's part of a section field to <prefix><rank><suffix>
+  subroutine write_boxes(prefix, suffix, g)
+    character(len=*), intent(in) :: prefix, suffix
+    real, intent(in) :: g(:)
+    real, allocatable :: a(:)
+    character(len=40) :: sr
+    integer :: bb
+    do bb = 1, pp
+       write(sr, '(I0)') bb - 1
+       allocate(a(box_points(bb)))
+       call take(a, g, bb)
+       call write_field(prefix//trim(sr)//suffix, a)
+       deallocate(a)
+    end do
+  end subroutine write_boxes
 
   subroutine read_field(fn, a)
     character(len=*), intent(in) :: fn

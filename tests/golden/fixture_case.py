@@ -27,10 +27,16 @@ CASES = {
     "force_theta_3d": ((5, 4, 3), 40, 0.2, 0, 1, (50.6, 19.4, 20000.0), 3, False),
     "force_phi_3d": ((4, 5, 3), 40, 0.2, 1, 2, (49.5, 20.6, 35000.0), 3, False),
     "anelastic_3d_adjoint": ((4, 4, 3), 47, 0.2, 1, 10, (50.3, 20.2, 45000.0), 3, True),
+    # split into 2 x 1 x 2 rank boxes by tests/golden/make_golden_multibox.py (elastic: the
+    # reference's multi-rank anelastic runs leave sum(D_p) at 0 on ranks > 0, since
+    # SOURCE/ses3d_input.f90:353 forms it on rank 0 only and :398-399 broadcast D_p but not the sum)
+    "multibox_adjoint": ((6, 5, 4), 45, 0.2, 0, 10, (48.3, 19.1, 60000.0), 3, True),
 }
 # cases with a laterally varying model: seeded Gaussian-correlated perturbation
 # (200 km correlation, 3% rms) of vp, vs and rho, and of Q (x3) when anelastic
-MODEL_3D = ("force_theta_3d", "force_phi_3d", "anelastic_3d_adjoint")
+MODEL_3D = ("force_theta_3d", "force_phi_3d", "anelastic_3d_adjoint", "multibox_adjoint")
+# cases whose fixture comes from a multi-rank reference run (not make_golden.py)
+MULTIBOX = {"multibox_adjoint": (2, 1, 2)}
 MODEL_FIELDS = ("rhoinv", "mu", "lambda", "Q")
 
 
@@ -70,8 +76,10 @@ def compact_inputs(name: str, seed: int = 11) -> dict:
     return out
 
 
-def write_tree(fx: dict, root: str, adjoint_flag: int = None) -> None:
-    """Write the run tree for a (loaded or fresh) compact fixture."""
+def write_tree(fx: dict, root: str, adjoint_flag: int = None, decomp=(1, 1, 1)) -> None:
+    """Write the run tree for a (loaded or fresh) compact fixture; ``decomp``
+    (px, py, pz) splits the section into that many rank boxes, each with its
+    own model records, as MODELS/SOURCE/generate_models.f90 lays them out."""
     nel = tuple(int(v) for v in fx["nel"])
     nt, dt = int(fx["nt"]), float(fx["dt"])
     is_diss = int(fx["is_diss"])
@@ -80,6 +88,8 @@ def write_tree(fx: dict, root: str, adjoint_flag: int = None) -> None:
     setup = M.section_setup(*nel, is_diss=bool(is_diss), adjoint_flag=af, samp_ad=int(fx["samp_ad"]))
     box = F.make_boxes(setup)[0]
     shape = (box.nx + 1, box.ny + 1, box.nz + 1, 5, 5, 5)
+    if tuple(decomp) != (1, 1, 1):
+        setup = M.decompose(setup, *decomp)
 
     def bz(v):
         return np.asfortranarray(np.broadcast_to(np.asarray(v, np.float32)[None, None, :, None, None, :], shape))
