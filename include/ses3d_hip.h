@@ -75,6 +75,16 @@ int ses3d_hip_set_model(ses3d_hip_context *ctx, const float *rhoinv, const float
                         const float *lambda, const float *A, const float *B, const float *C,
                         const float *Q);
 
+/* ses3d_hip_set_model from device memory: the same fields, float32, reference
+ * layout, at device addresses readable from the context's device (e.g. the
+ * buffers of torch tensors), read on the context's stream without a host
+ * round trip.  Their contents must be complete when the call is made (the
+ * call orders nothing against the caller's streams); it returns once they
+ * have been read. */
+int ses3d_hip_set_model_device(ses3d_hip_context *ctx, const float *rhoinv, const float *mu,
+                               const float *lambda, const float *A, const float *B, const float *C,
+                               const float *Q);
+
 /* Source of event ev: source_type 1/2/3 (point force theta/phi/r) or 10
  * (moment tensor); xxs, yys in rad, zzs depth in m; mom = (Mtt,Mpp,Mrr,
  * Mtp,Mtr,Mpr); so = nt samples of the source time function.  Replaces the
@@ -146,6 +156,12 @@ int ses3d_hip_get_field(ses3d_hip_context *ctx, int32_t ev, const char *name, fl
  * SOURCE/ses3d_output.f90:318-343. */
 int ses3d_hip_get_kernels(ses3d_hip_context *ctx, int32_t ev, double *grad_rho, double *grad_cp,
                           double *grad_csh, double *grad_csv, double *grad_Q_mu);
+
+/* ses3d_hip_get_kernels into device memory: the same float64 values in the
+ * same reference layout, written on the context's stream to device buffers
+ * of NPTS doubles (any may be NULL); returns once they are written. */
+int ses3d_hip_get_kernels_device(ses3d_hip_context *ctx, int32_t ev, double *grad_rho, double *grad_cp,
+                                 double *grad_csh, double *grad_csv, double *grad_Q_mu);
 
 /* Forward-field history (adjoint_flag==1 writes it, adjoint_flag==2 reads
  * it).  A stored step holds 9 float32 fields in reference layout, in the

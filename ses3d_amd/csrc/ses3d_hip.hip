@@ -1042,6 +1042,34 @@ __global__ void k_from_ref(T *__restrict__ dst, const T *__restrict__ src, long 
     dst[(e * EST + nd) * stride] = src[i];
 }
 
+// float32 accumulators widened into a float64 reference-layout array
+__global__ void k_to_ref_widen(double *__restrict__ dst, const float *__restrict__ src, long long NE, int stride) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NE * 125) return;
+    const long long nd = i / NE, e = i - nd * NE;
+    dst[i] = (double)src[(e * EST + nd) * stride];
+}
+
+// grad_Q_mu <- 2 mu grad_Q_mu / Q at it == nt (SOURCE/ses3d_grad_Q.f90:151-153),
+// in the accumulators' type; g in reference layout, mu and Q element-major
+__global__ void k_q_normalise(double *__restrict__ g, const float *__restrict__ lm, const float *__restrict__ Q,
+                              long long NE, int f32) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NE * 125) return;
+    const long long nd = i / NE, e = i - nd * NE, p = e * EST + nd;
+    const float mu = lm[2 * p + 1], q = Q[p];
+    if (f32) g[i] = (double)(((2.0f * mu) * (float)g[i]) / q);
+    else g[i] = ((2.0 * (double)mu) * g[i]) / (double)q;
+}
+
+// flag <- 1 if any of a, b, c (n values each) is nonzero
+__global__ void k_any_nonzero(const float *__restrict__ a, const float *__restrict__ b, const float *__restrict__ c,
+                              long long n, int *flag) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (a[i] != 0.0f || b[i] != 0.0f || c[i] != 0.0f) atomicOr(flag, 1);
+}
+
 __global__ void k_set_step(int *itp, int it) {
     if (threadIdx.x == 0) *itp = it;
 }
@@ -1271,6 +1299,7 @@ struct ses3d_hip_context : StepRunner {
     unsigned long long *div_buf = nullptr;        // divisors, reciprocals, failure counts
     float *conv = nullptr;                         // layout-conversion scratch for downloads
     size_t conv_cap = 0;
+    int *flag_d = nullptr;                         // device-side anisotropy test of set_model_device
 
     int launch_receivers_on(hipStream_t st, bool record);
     int launch_forces_on(hipStream_t st);
@@ -2739,6 +2768,61 @@ int ses3d_hip_get_kernels(ses3d_hip_context *c, int32_t ev, double *g_rho, doubl
         else
             for (long long i = 0; i < c->NPTS; ++i) g_Q[i] = ((2.0 * (double)mur[i]) * g_Q[i]) / (double)qr[i];
     }
+    return 0;
+}
+
+int ses3d_hip_set_model_device(ses3d_hip_context *c, const float *rhoinv, const float *mu, const float *lambda,
+                               const float *A, const float *B, const float *C, const float *Q) {
+    if (!c || !rhoinv || !mu || !lambda || !A || !B || !C) return fail(SES3D_HIP_EINVAL, "null model field");
+    if (c->cfg.is_diss && !Q) return fail(SES3D_HIP_EINVAL, "is_diss=1 needs Q");
+    const long long n = c->NPTS;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    const float *src[5] = {rhoinv, A, B, C, Q};
+    float *dst[5] = {c->rhoinv, c->A, c->B, c->C, c->Q};
+    for (int f = 0; f < 5; ++f)
+        if (dst[f] && src[f]) hipLaunchKernelGGL(k_from_ref<float>, grid, block, 0, c->stream, dst[f], src[f], c->NE, 1);
+    hipLaunchKernelGGL(k_from_ref<float>, grid, block, 0, c->stream, c->lm, lambda, c->NE, 2);   // (lambda, mu) pairs
+    hipLaunchKernelGGL(k_from_ref<float>, grid, block, 0, c->stream, c->lm + 1, mu, c->NE, 2);
+    HIP_TRY(hipGetLastError());
+    // anisotropy test on the device (A, B, C all zero: isotropic kernels)
+    if (!c->flag_d) TRY(dalloc(c, &c->flag_d, 1));
+    HIP_TRY(hipMemsetAsync(c->flag_d, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_any_nonzero, grid, block, 0, c->stream, A, B, C, n, c->flag_d);
+    HIP_TRY(hipGetLastError());
+    int any = 0;
+    TRY(download(c, &any, c->flag_d, sizeof(int)));   // also: the caller's buffers are no longer read
+    c->aniso = any != 0;
+    c->model_set = true;
+    c->initialised = false;
+    invalidate_checkpoints(c);
+    return 0;
+}
+
+int ses3d_hip_get_kernels_device(ses3d_hip_context *c, int32_t ev, double *g_rho, double *g_cp, double *g_csh,
+                                 double *g_csv, double *g_Q) {
+    TRY(check_ev(c, ev));
+    EventHost &E = c->evh[ev];
+    if (!E.grad) return fail(SES3D_HIP_ESTATE, "kernels exist only for adjoint_flag=2 contexts");
+    if (g_Q && !c->cfg.is_diss) return fail(SES3D_HIP_EINVAL, "grad_Q_mu needs is_diss=1");
+    const long long NP = c->NP, n = c->NPTS;
+    const bool f32 = c->kernel_bits == 32;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    double *outs[5] = {g_rho, g_cp, g_csh, g_csv, g_Q};
+    for (int k = 0; k < 5; ++k) {
+        if (!outs[k]) continue;
+        // pair planes (rho, cp) (csh, csv), then Q_mu
+        const long long off = k < 4 ? (k / 2) * 2 * NP + (k % 2) : 4 * NP;
+        const int stride = k < 4 ? 2 : 1;
+        if (f32)
+            hipLaunchKernelGGL(k_to_ref_widen, grid, block, 0, c->stream, outs[k], (const float *)E.grad + off, c->NE,
+                               stride);
+        else
+            hipLaunchKernelGGL(k_to_ref<double>, grid, block, 0, c->stream, outs[k], (const double *)E.grad + off,
+                               c->NE, stride);
+    }
+    if (g_Q) hipLaunchKernelGGL(k_q_normalise, grid, block, 0, c->stream, g_Q, c->lm, c->Q, c->NE, f32 ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 

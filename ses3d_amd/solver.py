@@ -106,6 +106,8 @@ def load_library(path: str = LIB_PATH):
         "ses3d_hip_group_get_timing": ([P, dp, dp, dp, ip], C.c_int),
         "ses3d_hip_project": ([P, fp, i32, fp, i32, fp, i32, fp, fp], C.c_int),
         "ses3d_hip_project_noalloc": ([P, fp, i32, fp, i32, fp, i32, fp, fp], C.c_int),
+        "ses3d_hip_set_model_device": ([P] + [C.c_void_p] * 7, C.c_int),
+        "ses3d_hip_get_kernels_device": ([P, i32] + [C.c_void_p] * 5, C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -188,6 +190,38 @@ class Solver:
         arrs = [_ref_f32(model[k]) for k in ("rhoinv", "mu", "lambda", "A", "B", "C")]
         q = _ref_f32(model["Q"]) if (self.cfg.is_diss and "Q" in model) else None
         _check(self.lib.ses3d_hip_set_model(self.h, *[_fptr(a) for a in arrs], _fptr(q) if q is not None else None))
+
+    def set_model_tensors(self, model: dict):
+        """set_model from torch tensors on this context's GPU, indexed as the
+        reference arrays (nx+1, ny+1, nz+1, 5, 5, 5): they are brought into the
+        reference's memory layout on the device and read there
+        (ses3d_hip_set_model_device), with no host round trip."""
+        import torch
+        names = ("rhoinv", "mu", "lambda", "A", "B", "C") + (("Q",) if self.cfg.is_diss else ())
+        ref = {}
+        for k in names:
+            t = model[k]
+            if t.dtype != torch.float32 or not t.is_cuda or tuple(t.shape) != self.shape_local:
+                raise ValueError(f"{k}: a float32 GPU tensor of shape {self.shape_local} is needed")
+            ref[k] = t.permute(5, 4, 3, 2, 1, 0).contiguous()   # Fortran order of the reference arrays
+        torch.cuda.current_stream().synchronize()                # their contents are complete
+        ptr = [ref[k].data_ptr() for k in ("rhoinv", "mu", "lambda", "A", "B", "C")]
+        _check(self.lib.ses3d_hip_set_model_device(self.h, *ptr, ref["Q"].data_ptr() if "Q" in ref else None))
+
+    def kernels_tensors(self, ev: int = 0, with_q: Optional[bool] = None) -> dict:
+        """kernels() as float64 torch tensors on this context's GPU, indexed as
+        the reference arrays, written there by ses3d_hip_get_kernels_device."""
+        import torch
+        if with_q is None:
+            with_q = bool(self.cfg.is_diss)
+        names = ["grad_rho", "grad_cp", "grad_csh", "grad_csv"] + (["grad_Q_mu"] if with_q else [])
+        rev = tuple(reversed(self.shape_local))
+        outs = {n: torch.empty(rev, dtype=torch.float64, device="cuda") for n in names}
+        torch.cuda.current_stream().synchronize()
+        ptrs = [outs[n].data_ptr() if n in outs else None
+                for n in ("grad_rho", "grad_cp", "grad_csh", "grad_csv", "grad_Q_mu")]
+        _check(self.lib.ses3d_hip_get_kernels_device(self.h, ev, *ptrs))
+        return {n: t.permute(5, 4, 3, 2, 1, 0) for n, t in outs.items()}
 
     def set_source(self, ev: int, source_type: int, xxs, yys, zzs, moment, so):
         mom = np.ascontiguousarray(moment, np.float32)
