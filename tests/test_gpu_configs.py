@@ -1,0 +1,219 @@
+"""The benchmark configurations at their workloads (nt truncated where the
+oracle or HBM would not finish or fit), through size-independent properties.
+
+* config 1 (elastic 24x24x12, nt=2000): all 2000 steps bit-identical to the
+  float32 oracle (digests in tests/golden/elastic_24x24x12_oracle.npz; the
+  oracle reproduces the reference's float32 build bit for bit).
+* config 3 (gradient on 48x48x24, anelastic): kernels bit-identical for
+  checkpoint intervals k = 50, 100, 200 and with every segment's sampled
+  steps kept (nothing replayed); the gradient against central finite
+  differences along 5 seeded random directions (h = 1e-3 relative).
+* config 4 (32 events on 64x64x32 plus 6 uneven sections in one group): each
+  member after 30 grouped steps bit-identical to itself run alone.
+* config 5 (16 events on 64x64x32 through the torch.autograd.Function, in
+  sub-batches of 2 that fit HBM): the summed gradient equals the sum of the
+  single-event gradients bit for bit, and one steepest-descent update lowers
+  the misfit of plain single-event solves.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+from dataclasses import replace
+
+from ses3d_amd import configs
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _solver(run, af, w, nt=None, n=None):
+    from ses3d_amd.solver import Solver
+    s = Solver.from_run(run, adjoint_flag=af, pml=w.pml, nt=nt, n_events=n)
+    s.set_model(run.model)
+    return s
+
+
+def test_config1_elastic_24x24x12_all_steps_match_oracle():
+    from tests.golden.fixture_case import field_digest, input_digest
+    fx = np.load(os.path.join(GOLDEN, "elastic_24x24x12_oracle.npz"))
+    w = configs.WORKLOADS["elastic_24x24x12"]
+    run = configs.build(w, seed=0)
+    assert input_digest(run) == str(fx["digest"]), "inputs drifted from the fixture's"
+    nt = int(fx["nt"])
+    s = _solver(run, 0, w)
+    s.load_event(0, run.events[0])
+    s.init()
+    print(f"config 1: {nt} steps", flush=True)
+    s.run(1, nt - 1)
+    snap = {c: s.field(c) for c in ("vx", "vy", "vz")}
+    s.run(nt, 1)
+    seis = s.seismograms()
+    s.close()
+    assert hashlib.sha256(np.ascontiguousarray(seis, np.float32).tobytes()).hexdigest() == str(fx["seismograms_digest"])
+    for c in ("vx", "vy", "vz"):
+        assert field_digest(snap[c]) == str(fx[c + "_digest"]), c
+
+
+def _gradient_case(nt):
+    w = configs.WORKLOADS["gradient_48x48x24"]
+    true = configs.build(w, seed=0, nt=nt)
+    cur = configs.build(replace(w, perturbation="none"), seed=0, nt=nt)
+    s = _solver(true, 0, w, nt=nt)
+    s.load_event(0, true.events[0])
+    s.init()
+    s.run()
+    data = s.seismograms()
+    s.close()
+    return w, cur, data
+
+
+def test_config3_kernels_same_for_every_checkpoint_interval():
+    from ses3d_amd import misfit as MF
+    nt = 400
+    w, cur, data = _gradient_case(nt)
+    e = cur.events[0]
+    f = _solver(cur, 1, w, nt=nt)
+    f.load_event(0, e)
+    a = _solver(cur, 2, w, nt=nt)
+    ref = None
+    for k, keep in ((100, 1), (50, 1), (200, 1), (50, nt // 50)):
+        print(f"config 3: checkpoints every {k} steps, {keep} segment(s) kept", flush=True)
+        f.init()
+        f.forward_checkpointed(k, keep=keep)
+        u = f.seismograms()
+        rows = MF.l2_adjoint_rows(u, data, float(e.dt))
+        a.set_adjoint_sources(0, e.rec32[f.receiver_index(0)], rows)
+        a.init()
+        a.adjoint_checkpointed(f)
+        K = a.kernels()
+        if ref is None:
+            ref = K
+            assert all(np.abs(v).max() > 0 for v in K.values())
+        else:
+            for g in ref:
+                np.testing.assert_array_equal(K[g], ref[g], err_msg=f"{g} k={k} keep={keep}")
+    f.close()
+    a.close()
+
+
+def test_config3_gradient_vs_finite_differences():
+    """Along 5 seeded directions (a Gaussian random field of 300 km correlation
+    each, the same relative perturbation of rho, vp, vsh, vsv) the adjoint gradient
+    and central differences of the discrete misfit (h = 1e-3) agree to what
+    the reference's kernel formulas allow: the O(dt) pairing of stored and
+    adjoint fields (the density term) and the elastic moduli in the anelastic
+    kernels (a bias of order sum(D_p)/Q on the shear terms).  Measured on this
+    workload: 0.26-0.48% for the five directions (the results are bit-for-bit
+    reproducible); the bar is 1%, which a wrong factor, sign or quadrature
+    weight in the kernel-to-gradient map would exceed."""
+    import torch
+    from ses3d_amd import models as M
+    from ses3d_amd.autograd import Problem, misfit
+    nt = 800
+    w, cur, data = _gradient_case(nt)
+    prob = Problem(cur, [data], pml=w.pml, ckpt=100, taper_width=20.0)
+    m = cur.model
+    rho0 = 1.0 / m["rhoinv"].astype(np.float64)
+    vs0 = np.sqrt(m["mu"] / rho0)
+    vp0 = np.sqrt((m["lambda"] + 2 * m["mu"]) / rho0)
+    base0 = [rho0, vp0, vs0, vs0.copy()]
+    x = [torch.tensor(a, requires_grad=True) for a in base0]
+    chi = misfit(prob, *x)
+    chi.backward()
+    grads = [t.grad.numpy() for t in x]
+    h = 1e-3
+    for d in range(5):
+        field = M.gaussian_random_field(cur.box, 500 + d, corr_len=300e3, rms=1.0)
+        dirs = [field * base0[k] for k in range(4)]
+        pred = sum(float(np.sum(grads[k] * dirs[k])) for k in range(4))
+        vals = [float(misfit(prob, *[torch.tensor(base0[k] + sgn * h * dirs[k]) for k in range(4)]))
+                for sgn in (1, -1)]
+        fd = (vals[0] - vals[1]) / (2 * h)
+        rel = abs(pred - fd) / abs(fd)
+        print(f"config 3 direction {d}: fd={fd:.6e} adjoint={pred:.6e} rel.err={rel:.2e}", flush=True)
+        assert np.sign(fd) == np.sign(pred) and rel <= 0.01, (d, fd, pred)
+
+
+def test_config4_group_members_equal_themselves_alone():
+    from ses3d_amd.solver import Group
+    w = configs.WORKLOADS["batched_64x64x32"]
+    steps = 30
+    runs = [(configs.build(w, seed=0, nt=steps), w)]
+    for k, nel in enumerate(configs.EXTRA_SECTIONS):
+        ws = configs.section_workload(nel, nt=steps, seed=k)
+        runs.append((configs.build(ws, seed=10 + k, nt=steps), ws))
+
+    def make(run, ww, events):
+        s = _solver(run, 0, ww, nt=steps, n=len(events))
+        for j, ev in enumerate(events):
+            s.load_event(j, run.events[ev])
+        s.init()
+        return s
+
+    def digest(s, j):
+        h = hashlib.sha256(s.seismograms(j).tobytes())
+        for c in ("vx_global", "vy_global", "vz_global"):
+            h.update(s.field(c, j).tobytes())
+        return h.hexdigest()
+
+    sols = [make(runs[0][0], w, range(w.n_events))] + [make(r, ww, [0]) for r, ww in runs[1:]]
+    g = Group(sols)
+    print(f"config 4: {w.n_events} events + {len(sols) - 1} sections, {steps} grouped steps", flush=True)
+    g.run(1, steps)
+    got = [[digest(s, j) for j in range(s.n_events)] for s in sols]
+    g.close()
+    for s in sols:
+        s.close()
+    for i, (run, ww) in enumerate(runs):
+        for j in range(len(got[i])):
+            s = make(run, ww, [j])
+            s.run(1, steps)
+            assert digest(s, 0) == got[i][j], (i, j)
+            s.close()
+
+
+def test_config5_batched_gradient_equals_sum_of_single_event_gradients():
+    import torch
+    from ses3d_amd.autograd import Problem, misfit
+    from ses3d_amd.solver import Solver
+    nt, n_events = 500, 16
+    w = configs.Workload("inversion_64x64x32", (64, 64, 32), True, 2, 0.1, nt, 200, "blobs", n_events=n_events)
+    true = configs.build(w, seed=0)
+    cur = configs.build(replace(w, perturbation="none"), seed=0)
+    f0 = Solver.from_run(true, adjoint_flag=0, pml=w.pml, n_events=n_events)
+    f0.set_model(true.model)
+    for j in range(n_events):
+        f0.load_event(j, true.events[j])
+    f0.init()
+    f0.run()
+    data = [f0.seismograms(j) for j in range(n_events)]
+    f0.close()
+    del true
+    m = cur.model
+    rho0 = 1.0 / m["rhoinv"].astype(np.float64)
+    vs0 = np.sqrt(m["mu"] / rho0)
+    vp0 = np.sqrt((m["lambda"] + 2 * m["mu"]) / rho0)
+    # 16 events through the autograd function, 2 per solve (checkpoints every 50 steps)
+    batched = Problem(cur, data, pml=w.pml, ckpt=50, events_per_batch=2, taper_width=10.0)
+    print(f"config 5: {n_events} events, nt={nt}, 2 per solve", flush=True)
+    x = [torch.tensor(a, requires_grad=True) for a in (rho0, vp0, vs0, vs0.copy())]
+    chi0 = misfit(batched, *x)
+    chi0.backward()
+    g = [t.grad.numpy() for t in x]
+    # the sum of the 16 single-event gradients, in event order
+    single = Problem(cur, data, pml=w.pml, ckpt=50, events_per_batch=1, taper_width=10.0)
+    model = single.fields(rho0, vp0, vs0, vs0)
+    c1, K1 = single.misfit_and_gradient(model)
+    g1 = single.gradient_from_kernels(K1, model)
+    assert c1 == float(chi0.detach())
+    for name, gb in zip(("rho", "vp", "vsh", "vsv"), g):
+        np.testing.assert_array_equal(gb, g1[name], err_msg=name)
+    # one steepest-descent update of the velocities lowers the misfit of single-event solves
+    scale = [0.0, 0.005 * vp0.mean(), 0.005 * vs0.mean(), 0.005 * vs0.mean()]
+    new = [x[i].detach().numpy() - scale[i] * g[i] / max(np.abs(g[i]).max(), 1e-300) for i in range(4)]
+    chi_before = single.misfit(single.fields(rho0, vp0, vs0, vs0))
+    chi_after = single.misfit(single.fields(*new))
+    print(f"config 5: misfit {chi_before:.6e} -> {chi_after:.6e}", flush=True)
+    assert chi_before == c1 and chi_after < chi_before
