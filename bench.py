@@ -90,15 +90,32 @@ def device_copy_gbps(mib: int = 2048, reps: int = 10) -> float:
     return 2 * 4 * n / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
-    """Time the reference binary on its largest one-rank mesh (its static
-    arrays hold 23 x 28 x 8 elements, SOURCE/ses3d_modules.f90:13-15);
+def host_cpu() -> dict:
+    """The host's CPU model, logical CPUs and the CPUs this process may use."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
+def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0, decomp=(1, 1, 1)):
+    """Time the reference binary on px*py*pz MPI ranks (decomp), each rank
+    holding the largest box its static arrays allow (23 x 28 x 8 elements,
+    SOURCE/ses3d_modules.f90:13-15), so the section is 23px x 28py x 8pz;
     per-step cost = difference of two runs of different length."""
     from oracle import pyoracle as PO
     from ses3d_amd import formats as F, models as M
     if not PO.ref_available():
         return None
-    nel = (23, 28, 8)
+    nprocs = int(np.prod(decomp))
+    nel = (23 * decomp[0], 28 * decomp[1], 8 * decomp[2])
     times = []
     for nt in steps:
         root = tempfile.mkdtemp(prefix="ses3d_cpu_")
@@ -111,9 +128,11 @@ def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
             ev = F.Event(nt=nt, dt=0.1, xxs=50.0, yys=20.0, zzs=10000.0, source_type=10,
                          moment=tuple(M.random_moment_tensor(rng)), ofd="../DATA/OUTPUT/1/", ssamp=10 ** 6)
             rec = M.surface_receiver_grid(setup, 10, 10)
+            if nprocs > 1:
+                setup = M.decompose(setup, *decomp)
             M.write_run_tree(root, M.RunCase(setup, [ev], [rec], M.ricker(nt + 4, 0.1, 20.0), model))
             t0 = time.perf_counter()
-            r = PO.run_ref(root, timeout=timeout)
+            r = PO.run_ref(root, timeout=timeout, nprocs=nprocs)
             t1 = time.perf_counter()
             if r.returncode != 0:
                 return None
@@ -122,11 +141,89 @@ def cpu_baseline_reference(is_diss: bool, steps=(10, 50), timeout=300.0):
             shutil.rmtree(root, ignore_errors=True)
     n_p = int(np.prod(nel)) * 125
     per_step = (times[1] - times[0]) / (steps[1] - steps[0])
-    return {"value": n_p / per_step, "unit": "GLL-pt updates/s", "cores": 1, "kind": "reference",
-            "sample": (f"reference Fortran (amdflang -O3, 1 MPI rank) on {nel[0]}x{nel[1]}x{nel[2]} "
-                       f"{'anelastic' if is_diss else 'elastic'} elements (its compile-time maximum per "
-                       f"rank), per-step time = difference of {steps[1]}- and {steps[0]}-step runs "
-                       f"({times[1]:.1f} s and {times[0]:.1f} s)")}
+    ranks = f"{nprocs} MPI ranks ({decomp[0]}x{decomp[1]}x{decomp[2]} boxes of 23x28x8)" if nprocs > 1 else \
+        "1 MPI rank"
+    return {"value": n_p / per_step, "unit": "GLL-pt updates/s", "cores": nprocs, "kind": "reference",
+            "sample": (f"reference Fortran (amdflang -O3, MPICH) on {ranks}: {nel[0]}x{nel[1]}x{nel[2]} "
+                       f"{'anelastic' if is_diss else 'elastic'} elements, per-step time = difference of "
+                       f"{steps[1]}- and {steps[0]}-step runs ({times[1]:.1f} s and {times[0]:.1f} s)"),
+            **host_cpu()}
+
+
+def source_sha256() -> str:
+    """sha256 of the HIP source the library is built from (stamps measured
+    counter files with the code they describe)."""
+    import hashlib
+    with open(os.path.join(ROOT, "ses3d_amd", "csrc", "ses3d_hip.hip"), "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()
+
+
+def traffic_file(workload: str) -> str:
+    return os.path.join(ROOT, "profiles", f"k_element_{workload}_traffic.json")
+
+
+def _pmc_mean_kib(csv_path: str, kernel: str) -> float:
+    import csv
+    vals = [float(r["Counter_Value"]) for r in csv.DictReader(open(csv_path)) if kernel in r["Kernel_Name"]]
+    if not vals:
+        raise RuntimeError(f"no {kernel} dispatches in {csv_path}")
+    return sum(vals) / len(vals)
+
+
+def measure_traffic(workload: str, steps: int = 8) -> dict:
+    """HBM bytes per element-kernel launch from two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over a short run of the
+    workload, each scaled by what the same counter reads for a plain device
+    copy of known size; written with the HIP source's sha256 to the
+    workload's traffic file, which the benchmark reports only while that
+    source is unchanged."""
+    import subprocess
+    out = tempfile.mkdtemp(prefix="ses3d_pmc_")
+    py = sys.executable
+    runs = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        for tag, prog in (("kernel", [py, os.path.abspath(__file__), "--traffic-probe", workload,
+                                      "--steps", str(steps)]),
+                          ("copy", [py, os.path.abspath(__file__), "--calibrate-copy", "1024"])):
+            d = os.path.join(out, f"{tag}_{counter}")
+            cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--", *prog]
+            print("# " + " ".join(cmd), file=sys.stderr, flush=True)
+            subprocess.run(cmd, check=True, timeout=300, cwd=out, stdout=subprocess.DEVNULL)
+            runs[(tag, counter)] = os.path.join(d, "run_counter_collection.csv")
+    copy_bytes = 1024 << 20
+    fetch_scale = copy_bytes / 1024 / _pmc_mean_kib(runs[("copy", "FETCH_SIZE")], "copyBuffer")
+    write_scale = copy_bytes / 1024 / _pmc_mean_kib(runs[("copy", "WRITE_SIZE")], "copyBuffer")
+    fetch = _pmc_mean_kib(runs[("kernel", "FETCH_SIZE")], "k_element")
+    write = _pmc_mean_kib(runs[("kernel", "WRITE_SIZE")], "k_element")
+    nfetch = _pmc_mean_kib(runs[("kernel", "FETCH_SIZE")], "k_node")
+    nwrite = _pmc_mean_kib(runs[("kernel", "WRITE_SIZE")], "k_node")
+    res = {"kernel": "k_element", "workload": workload,
+           "hbm_bytes_per_launch": 1024 * (fetch * fetch_scale + write * write_scale),
+           "fetch_kib": fetch, "write_kib": write, "fetch_scale": fetch_scale, "write_scale": write_scale,
+           "node_kernel_bytes_per_launch": 1024 * (nfetch * fetch_scale + nwrite * write_scale),
+           "source_sha256": source_sha256(),
+           "command": f"python bench.py --measure-traffic {workload} (rocprofv3 --pmc FETCH_SIZE, then "
+                      f"WRITE_SIZE, on bench.py --traffic-probe {workload} --steps {steps} and on a "
+                      f"1 GiB device copy)"}
+    with open(traffic_file(workload), "w") as fh:
+        json.dump(res, fh, indent=1)
+    shutil.rmtree(out, ignore_errors=True)
+    return res
+
+
+def traffic_probe(workload: str, steps: int) -> None:
+    """A short forward run of the workload for counter passes."""
+    from ses3d_amd import configs
+    from ses3d_amd.solver import Solver
+    w = configs.WORKLOADS[workload]
+    run = configs.build(w, seed=0, nt=steps)
+    s = Solver.from_run(run, adjoint_flag=0, pml=w.pml, nt=steps)
+    s.set_model(run.model)
+    s.load_event(0, run.events[0])
+    s.init()
+    s.run()
+    s.synchronize()
+    s.close()
 
 
 def configs_nt(name: str) -> int:
@@ -377,11 +474,26 @@ def main():
                     help="only run one inversion iteration on the 64x64x32 mesh (16 events, ~10 min)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run a device copy of this many MiB (for HBM counter calibration)")
+    ap.add_argument("--measure-traffic", metavar="WORKLOAD",
+                    help="only measure the element kernel's HBM bytes per launch with rocprofv3 counter "
+                         "passes and write them, stamped with the HIP source's sha256, under profiles/")
+    ap.add_argument("--traffic-probe", metavar="WORKLOAD", help=argparse.SUPPRESS)
+    ap.add_argument("--graph-steps", type=int, default=None,
+                    help="steps per captured step graph in every context (0: direct launches)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the whole section is resident on a single MI355X and this "
               "build runs no multi-GPU path (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.measure_traffic:
+        print(json.dumps(measure_traffic(args.measure_traffic)))
+        return
+    if args.graph_steps is not None:
+        from ses3d_amd.solver import Solver
+        Solver.default_graph_steps = args.graph_steps
+    if args.traffic_probe:
+        traffic_probe(args.traffic_probe, args.steps)
+        return
     import torch
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 4
@@ -434,11 +546,17 @@ def main():
     ab = algorithmic_bytes(w.nel, w.is_diss)
     elem_ms = tm["element_ms"] / max(tm["n_element"], 1)
     achieved = ab["element"] * n_p / (elem_ms * 1e-3)
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"k_element_{w.name}_traffic.json")
+    # counter-measured HBM bytes per element-kernel launch, reported only if
+    # measured on the HIP source this library is built from
+    traffic, traffic_note = None, "no counter file"
+    tfile = traffic_file(w.name)
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            tj = json.load(fh)
+        if tj.get("source_sha256") == source_sha256():
+            traffic, traffic_note = tj.get("hbm_bytes_per_launch"), "rocprofv3 PMC of this HIP source: " + tj["command"]
+        else:
+            traffic_note = "counter file measured on another HIP source: not reported"
     out = {
         "metric": METRIC, "value": value, "unit": "GLL-pt updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
@@ -454,7 +572,8 @@ def main():
                      "mean_launch_ms": elem_ms,
                      "impl_bytes_per_point": implemented_bytes(w.nel, w.pml, w.is_diss),
                      "frac_impl": implemented_bytes(w.nel, w.pml, w.is_diss) * n_p / (elem_ms * 1e-3) / HBM_PEAK,
-                     "traffic_frac": (traffic / (elem_ms * 1e-3) / HBM_PEAK) if traffic else None},
+                     "traffic_frac": (traffic / (elem_ms * 1e-3) / HBM_PEAK) if traffic else None,
+                     "traffic_source": traffic_note},
         "step_roofline_frac": value * ab["step"] / HBM_PEAK,
         "kernel_ms_per_step": {"element": elem_ms, "node": tm["node_ms"] / max(tm["n_element"], 1)},
         "schedule": sched,
@@ -507,8 +626,16 @@ def main():
         print("# batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg()
     if not args.no_cpu_baseline:
-        print("# timing the reference CPU solver", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline_reference(w.is_diss)
+        # the reference on 16 MPI ranks (4x4x1 boxes of its one-rank maximum), this
+        # box's CPU share, and on one rank beside it
+        print("# timing the reference CPU solver (16 ranks, then 1)", file=sys.stderr, flush=True)
+        cb = cpu_baseline_reference(w.is_diss, decomp=(4, 4, 1))
+        one = cpu_baseline_reference(w.is_diss)
+        if cb is None:
+            cb = one
+        elif one is not None:
+            cb["one_rank"] = {k: one[k] for k in ("value", "cores", "sample")}
+        out["cpu_baseline"] = cb
     print(json.dumps(out))
 
 

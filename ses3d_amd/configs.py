@@ -97,8 +97,9 @@ def build(w: Workload, seed: int = 0, nt: Optional[int] = None, n_events: Option
     for k in range(n_events):
         if k == 0:
             colat, lon, depth = 50.0, 20.0, 10000.0
-            side = int(round(np.sqrt(w.n_rec)))
-            rec = M.surface_receiver_grid(setup, side, max(1, w.n_rec // side), pml=w.pml)
+            # exactly n_rec receivers on the squarest grid: side = largest divisor <= sqrt(n_rec)
+            side = max(d for d in range(1, int(np.sqrt(w.n_rec)) + 1) if w.n_rec % d == 0)
+            rec = M.surface_receiver_grid(setup, side, w.n_rec // side, pml=w.pml)
         else:
             colat, lon = rng.uniform(42.0, 58.0), rng.uniform(12.0, 28.0)
             depth = rng.uniform(5e3, 600e3)

@@ -139,6 +139,9 @@ class Solver:
     Parameters are the float32 values the reference holds after ses3d_input:
     box bounds in rad (theta, phi) and m (radius), ``zmax_global`` in m."""
 
+    # steps per captured step graph for every new context (None: the library's default)
+    default_graph_steps: Optional[int] = None
+
     def __init__(self, *, nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax, zmax_global, nt, dt,
                  is_diss=0, adjoint_flag=0, samp_ad=1, pml=2, tau_p=(1.0, 1.0, 1.0),
                  D_p=(0.0, 0.0, 0.0), n_events=1):
@@ -149,6 +152,8 @@ class Solver:
         h = C.c_void_p()
         _check(self.lib.ses3d_hip_create(C.byref(self.cfg), n_events, C.byref(h)))
         self.h = h
+        if Solver.default_graph_steps is not None:
+            self.set_graph_steps(Solver.default_graph_steps)
         self.n_events = n_events
         self.nt = nt
         self.shape_local = (nx + 1, ny + 1, nz + 1, 5, 5, 5)
