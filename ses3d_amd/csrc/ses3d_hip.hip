@@ -121,6 +121,14 @@ struct MeshDev {
     long long hslot;
     // kernel accumulators in float32 (the reference's type) instead of float64
     int kf32;
+    int nt;
+};
+
+// One point force or adjoint source (SOURCE/ses3d_miscellaneous.f90:29-92).
+struct ForceDev {
+    int comp_mask;        // bit c: subtract from component c
+    const float *w;       // 125 weights
+    const float *stf;     // 3*nt: component c at stf[c*nt + it-1]
 };
 
 struct EventDev {
@@ -139,6 +147,11 @@ struct EventDev {
     double *grad;          // 5*NP  pair planes (rho,cp) (csh,csv), Q_mu at 4*NP; with
                            // float32 accumulators the same layout in floats
     float sdt;             // samp_ad*dt
+    // point forces / adjoint sources: per element {first, count} into
+    // `forces` (the element's sources in their reference order), or nullptr
+    // when the event has none
+    const int2 *force_elem;
+    const ForceDev *forces;
 };
 
 // Loads and stores of the streamed per-point state (touched once per step).
@@ -309,6 +322,40 @@ __device__ __forceinline__ long long gidx(int gx, int gy, int gz, int NBX, int N
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
     int q = nb >> 3, r = nb & 7, x = bid & 7, loc = bid >> 3;
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
+}
+
+// Point forces and adjoint sources of one element (SOURCE/ses3d_miscellaneous.f90:
+// 29-92, called at ses3d_evolution.f90:276): the element force at node nd
+// less w_nd * stf(it) of each source, in the sources' order.  Sources are
+// read in chunks of FCH (independent loads, one round trip), the
+// subtractions then run in order.
+template <typename FT>
+__device__ __forceinline__ void subtract_forces(FT *__restrict__ F, const int count, const int nd, const int nt,
+                                                const int it, float &fx, float &fy, float &fz) {
+    constexpr int FCH = 8;
+    for (int s0 = 0; s0 < count; s0 += FCH) {
+        float w[FCH], sx[FCH], sy[FCH], sz[FCH];
+        int mask[FCH];
+#pragma unroll
+        for (int j = 0; j < FCH; ++j) {
+            mask[j] = 0;
+            w[j] = sx[j] = sy[j] = sz[j] = 0.0f;
+            if (s0 + j < count) {
+                const ForceDev Fj = F[s0 + j];
+                mask[j] = Fj.comp_mask;
+                w[j] = Fj.w[nd];
+                if (Fj.comp_mask & 1) sx[j] = Fj.stf[0 * nt + it - 1];
+                if (Fj.comp_mask & 2) sy[j] = Fj.stf[1 * nt + it - 1];
+                if (Fj.comp_mask & 4) sz[j] = Fj.stf[2 * nt + it - 1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < FCH; ++j) {
+            if (mask[j] & 1) fx = fx - w[j] * sx[j];
+            if (mask[j] & 2) fy = fy - w[j] * sy[j];
+            if (mask[j] & 4) fz = fz - w[j] * sz[j];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -763,6 +810,11 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             ay = ((ay + ss[sub][czy][pz_] * wz) + ss[sub][EYY][py_] * wy) + ss[sub][EXY][px_] * wx;
             az = ((az + ss[sub][EZZ][pz_] * wz) + ss[sub][EYZ][py_] * wy) + ss[sub][EXZ][px_] * wx;
         }
+        // point forces / adjoint sources of this element, after the divergence
+        if (E.force_elem) {
+            const int2 fe = E.force_elem[e];
+            if (fe.y > 0 && nd < 125) subtract_forces(E.forces + fe.x, fe.y, nd, Ms.nt, it, ax, ay, az);
+        }
         // tail lanes write the slot's padding, real nodes their fperm position
         const unsigned fq = 4u * (unsigned)(e * EST + (nd < 125 ? fperm(l, m, n) : nd));
         stc2(E.f, 2u * fq, ax, ay);
@@ -815,64 +867,6 @@ void k_element_group(const SectionDev *__restrict__ secs, int nsec, int remap) {
     cSectionDev *cs = (cSectionDev *)secs;
     cSectionDev &S = cs[section_of(cs, nsec, b, false)];
     element_step<DISS, MODE, ANISO, EPB, NT>(*(cMeshDev *)S.mesh, (cEventDev *)S.evs, *S.itp, b - S.eb0, S.bpe);
-}
-
-// ---------------------------------------------------------------------------
-// point forces (forward source_type 1/2/3, adjoint sources): misc.f90:29-92.
-// One block per (event, source element); each thread one node; the element's
-// sources are applied in their reference order.
-// ---------------------------------------------------------------------------
-struct ForceGroup {
-    int ev;
-    long long elem;
-    int first, count;   // into the force list
-};
-struct ForceDev {
-    int comp_mask;        // bit c: subtract from component c
-    const float *w;       // 125 weights
-    const float *stf;     // 3*nt: component c at stf[c*nt + it-1]
-};
-
-__global__ void k_forces(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs,
-                         const ForceGroup *__restrict__ groups, const ForceDev *__restrict__ forces,
-                         int nt, const int *itp) {
-    const int it = *itp;
-    const ForceGroup G = groups[blockIdx.x];
-    const int nd = threadIdx.x;
-    if (nd >= 125) return;
-    const long long NP = mesh->NP;
-    float *f = evs[G.ev].f;
-    const long long p = G.elem * EST + fperm(nd % 5, (nd / 5) % 5, nd / 25);
-    float fx = f[2 * p], fy = f[2 * p + 1], fz = f[2 * NP + p];
-    // sources in chunks of FCH: the loads of a chunk are independent of each
-    // other (one round trip instead of one per source), the subtractions then
-    // run in the sources' order
-    constexpr int FCH = 8;
-    const int end = G.first + G.count;
-    for (int s0 = G.first; s0 < end; s0 += FCH) {
-        float w[FCH], sx[FCH], sy[FCH], sz[FCH];
-        int mask[FCH];
-#pragma unroll
-        for (int j = 0; j < FCH; ++j) {
-            mask[j] = 0;
-            w[j] = sx[j] = sy[j] = sz[j] = 0.0f;
-            if (s0 + j < end) {
-                const ForceDev F = forces[s0 + j];
-                mask[j] = F.comp_mask;
-                w[j] = F.w[nd];
-                if (F.comp_mask & 1) sx[j] = F.stf[0 * nt + it - 1];
-                if (F.comp_mask & 2) sy[j] = F.stf[1 * nt + it - 1];
-                if (F.comp_mask & 4) sz[j] = F.stf[2 * nt + it - 1];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < FCH; ++j) {
-            if (mask[j] & 1) fx = fx - w[j] * sx[j];
-            if (mask[j] & 2) fy = fy - w[j] * sy[j];
-            if (mask[j] & 4) fz = fz - w[j] * sz[j];
-        }
-    }
-    f[2 * p] = fx; f[2 * p + 1] = fy; f[2 * NP + p] = fz;
 }
 
 // ---------------------------------------------------------------------------
@@ -1202,7 +1196,6 @@ struct StepRunner {
     // the kernels of one step for every event, reading the step number from HBM
     virtual int launch_receivers(bool record) = 0;
     virtual int launch_element(int mode) = 0;
-    virtual int launch_forces() = 0;
     virtual int launch_node() = 0;
     virtual int set_step(int it) = 0;   // store it in every step counter, stream-ordered
     virtual int schedule_changed() {
@@ -1240,14 +1233,15 @@ struct EventHost {
     float *seis = nullptr;     // nr x 3 x nt
     float *rec_w = nullptr;    // nr x 125
     float *force_w = nullptr, *force_stf = nullptr;
+    int2 *force_elem = nullptr;   // per element {first, count} into the context's force list
+    size_t force_elem_cap = 0;
+    bool has_forces = false;
     size_t seis_cap = 0, rec_w_cap = 0, force_w_cap = 0, force_stf_cap = 0;
     double *grad = nullptr;
     float *hist = nullptr;       // sampled forward fields the kernels read/write (active buffer)
     float *hist_own = nullptr;   // full history of every sampled step (allocated on first use)
     int n_slots = 0;
     std::vector<int> slot_h;     // per forward step: index among the sampled steps, or -1
-    std::vector<int> slot_dev;   // the table currently on the device ([it] -> slot)
-    int *slot_d = nullptr;
     // restart checkpoints (forward_checkpointed) and the segment buffer replayed into
     std::vector<float *> ckpt;
     float *seg = nullptr;
@@ -1281,10 +1275,8 @@ struct ses3d_hip_context : StepRunner {
     // receivers / forces (all events)
     RecDev *recs_d = nullptr;
     int nrec_total = 0;
-    ForceGroup *groups_d = nullptr;
     ForceDev *forces_d = nullptr;
-    int n_groups = 0;
-    size_t recs_cap = 0, groups_cap = 0, forces_cap = 0;
+    size_t recs_cap = 0, forces_cap = 0;
     std::vector<int> saving_vector;   // 1..nt
     int ckpt_k = 0;
     int ckpt_keep = 1;       // trailing segments whose sampled steps the checkpointed forward keeps
@@ -1300,13 +1292,16 @@ struct ses3d_hip_context : StepRunner {
     float *conv = nullptr;                         // layout-conversion scratch for downloads
     size_t conv_cap = 0;
     int *flag_d = nullptr;                         // device-side anisotropy test of set_model_device
+    // [it] -> slot of the sampled-step buffers: the same table for every event
+    // of the context (their sampled steps are the same), uploaded once
+    int *slot_d = nullptr;
+    std::vector<int> slot_dev;                     // the table currently on the device
+    std::vector<EventDev> ev_dev;                  // the event table currently on the device
 
     int launch_receivers_on(hipStream_t st, bool record);
-    int launch_forces_on(hipStream_t st);
     int set_step_on(hipStream_t st, int it);
     int launch_receivers(bool record) override { return launch_receivers_on(stream, record); }
     int launch_element(int mode) override;
-    int launch_forces() override { return launch_forces_on(stream); }
     int launch_node() override;
     int set_step(int it) override { return set_step_on(stream, it); }
 };
@@ -1333,13 +1328,6 @@ struct ses3d_hip_group : StepRunner {
         return 0;
     }
     int launch_element(int mode) override;
-    int launch_forces() override {
-        for (auto *c : ctx) {
-            int rc = c->launch_forces_on(stream);
-            if (rc) return rc;
-        }
-        return 0;
-    }
     int launch_node() override;
     int set_step(int it) override {
         for (auto *c : ctx) {
@@ -1576,7 +1564,6 @@ int launch_step(StepRunner *r, int mode, bool record, bool timed) {
     }
     TRY(r->launch_element(mode));
     if (ev) HIP_TRY(hipEventRecord(ev[1], r->stream));
-    TRY(r->launch_forces());
     if (ev) HIP_TRY(hipEventRecord(ev[2], r->stream));
     TRY(r->launch_node());
     if (ev) {
@@ -1699,12 +1686,6 @@ int ses3d_hip_context::launch_receivers_on(hipStream_t st, bool record) {
     return 0;
 }
 
-int ses3d_hip_context::launch_forces_on(hipStream_t st) {
-    if (n_groups <= 0) return 0;
-    hipLaunchKernelGGL(k_forces, dim3(n_groups), dim3(128), 0, st, mesh_d, ev_d, groups_d, forces_d, cfg.nt, it_d);
-    HIP_TRY(hipGetLastError());
-    return 0;
-}
 
 int ses3d_hip_context::set_step_on(hipStream_t st, int it) {
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, st, it_d, it);
@@ -1837,7 +1818,7 @@ int build_slots(ses3d_hip_context *c, EventHost &E, bool allocate_history) {
     for (int it = 1; it <= nt; ++it)
         if ((it % sa) == 0 || it == 1 || it == nt) E.slot_h[it] = n++;
     E.n_slots = n;
-    if (!E.slot_d) TRY(dalloc(c, &E.slot_d, nt + 1));
+    if (!c->slot_d) TRY(dalloc(c, &c->slot_d, nt + 1));
     if (allocate_history && !E.hist_own) TRY(dalloc(c, &E.hist_own, (size_t)n * hist_slot_floats(c)));
     if (allocate_history) E.hist = E.hist_own;
     return 0;
@@ -1845,9 +1826,10 @@ int build_slots(ses3d_hip_context *c, EventHost &E, bool allocate_history) {
 
 // Device slot table of one event: fwd tables map it -> slot of step it,
 // adjoint tables map it -> slot of the paired forward step nt+1-it.
-int upload_slots(ses3d_hip_context *c, EventHost &E, const std::vector<int> &tab) {
-    E.slot_dev = tab;
-    HIP_TRY(hipMemcpyAsync(E.slot_d, E.slot_dev.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, c->stream));
+int upload_slots(ses3d_hip_context *c, const std::vector<int> &tab) {
+    if (tab == c->slot_dev) return 0;
+    c->slot_dev = tab;
+    HIP_TRY(hipMemcpyAsync(c->slot_d, c->slot_dev.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1864,7 +1846,7 @@ int ensure_full_history(ses3d_hip_context *c) {
         if (E.hist != E.hist_own) { E.hist = E.hist_own; changed = true; }
         std::vector<int> sl(nt + 1, -1);
         for (int it = 1; it <= nt; ++it) sl[it] = af == 1 ? E.slot_h[it] : E.slot_h[nt + 1 - it];
-        if (sl != E.slot_dev) TRY(upload_slots(c, E, sl));
+        TRY(upload_slots(c, sl));
     }
     if (changed) TRY(refresh_events(c));
     return 0;
@@ -1883,10 +1865,15 @@ int refresh_events(ses3d_hip_context *c) {
         D.src_md = E.src_md;
         D.so = E.so_d;
         D.hist = E.hist;
-        D.slot = E.slot_d;
+        D.slot = c->slot_d;
         D.grad = E.grad;
         D.sdt = (float)c->cfg.samp_ad * c->cfg.dt;
+        D.force_elem = E.has_forces ? E.force_elem : nullptr;
+        D.forces = E.has_forces ? c->forces_d : nullptr;
     }
+    if (c->ev_dev.size() == h.size() && std::memcmp(c->ev_dev.data(), h.data(), sizeof(EventDev) * h.size()) == 0)
+        return 0;   // the table on the device is this one
+    c->ev_dev = h;
     return upload(c, c->ev_d, h.data(), sizeof(EventDev) * c->n_events);
 }
 
@@ -2413,6 +2400,7 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     M.hnc = (c->hdeg + 1) * (c->hdeg + 1) * (c->hdeg + 1);
     M.hslot = hist_slot_floats(c);
     M.kf32 = c->kernel_bits == 32;
+    M.nt = c->cfg.nt;
     M.lm = c->lm; M.A = c->A; M.B = c->B; M.C = c->C; M.Q = c->Q; M.rhoinv = c->rhoinv;
     M.MM = c->MM; M.Z = c->Z; M.S = c->S; M.COT = c->COT; M.IX = c->IX; M.IY = c->IY; M.IZ = c->IZ;
     M.LUT = c->LUT;
@@ -2452,8 +2440,8 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     }
     // events: zero fields, locate source / receivers / adjoint sources
     std::vector<RecDev> recs;
-    std::vector<ForceGroup> groups;
-    std::vector<ForceDev> forces;
+    std::vector<ForceDev> forces;                 // every event's sources, grouped by element
+    std::vector<std::vector<int2>> force_elem(c->n_events);
     c->saving_vector.assign(nt + 1, 0);
     for (int ev = 0; ev < c->n_events; ++ev) {
         EventHost &E = c->evh[ev];
@@ -2529,9 +2517,9 @@ int ses3d_hip_init(ses3d_hip_context *c) {
             // group by element keeping source order within each element
             std::map<long long, std::vector<size_t>> by_elem;
             for (size_t k = 0; k < np; ++k) by_elem[plist[k].first].push_back(k);
+            force_elem[ev].assign((size_t)c->NE, int2{0, 0});
             for (auto &kv : by_elem) {
-                ForceGroup G;
-                G.ev = ev; G.elem = kv.first; G.first = (int)forces.size(); G.count = (int)kv.second.size();
+                force_elem[ev][(size_t)kv.first] = int2{(int)forces.size(), (int)kv.second.size()};
                 for (size_t k : kv.second) {
                     ForceDev F;
                     F.comp_mask = plist[k].second.first;
@@ -2539,14 +2527,14 @@ int ses3d_hip_init(ses3d_hip_context *c) {
                     F.stf = E.force_stf + k * 3 * (size_t)nt;
                     forces.push_back(F);
                 }
-                groups.push_back(G);
             }
         }
+        E.has_forces = !plist.empty();
         // sampled-step slots: adjoint_flag 1 stores, 2 reads the fwd step nt+1-it
         if (af == 1 || af == 2) {
             TRY(build_slots(c, E, false));
             E.hist = nullptr;
-            TRY(upload_slots(c, E, std::vector<int>(nt + 1, -1)));
+            TRY(upload_slots(c, std::vector<int>(nt + 1, -1)));
             for (int it = 1; it <= nt; ++it) c->saving_vector[it] = E.slot_h[it] >= 0 ? it : 0;
         }
     }
@@ -2555,12 +2543,15 @@ int ses3d_hip_init(ses3d_hip_context *c) {
         TRY(dfit(c, &c->recs_d, c->recs_cap, recs.size()));
         TRY(upload(c, c->recs_d, recs.data(), sizeof(RecDev) * recs.size()));
     }
-    c->n_groups = (int)groups.size();
-    if (!groups.empty()) {
-        TRY(dfit(c, &c->groups_d, c->groups_cap, groups.size()));
+    if (!forces.empty()) {
         TRY(dfit(c, &c->forces_d, c->forces_cap, forces.size()));
-        TRY(upload(c, c->groups_d, groups.data(), sizeof(ForceGroup) * groups.size()));
         TRY(upload(c, c->forces_d, forces.data(), sizeof(ForceDev) * forces.size()));
+        for (int ev = 0; ev < c->n_events; ++ev) {
+            EventHost &E = c->evh[ev];
+            if (!E.has_forces) continue;
+            TRY(dfit(c, &E.force_elem, E.force_elem_cap, (size_t)c->NE));
+            TRY(upload(c, E.force_elem, force_elem[ev].data(), sizeof(int2) * (size_t)c->NE));
+        }
     }
     TRY(refresh_events(c));
     c->drop_graphs();
@@ -2931,10 +2922,8 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
     const size_t nM = c->cfg.is_diss ? 18 * NP : 0;
     TRY(ensure_checkpoints(c, k, keep));
     // the forward pass keeps the displacement gradient (mode 1) but stores no sampled step
-    for (auto &E : c->evh) {
-        E.hist = nullptr;
-        TRY(upload_slots(c, E, std::vector<int>(nt + 1, -1)));
-    }
+    for (auto &E : c->evh) E.hist = nullptr;
+    TRY(upload_slots(c, std::vector<int>(nt + 1, -1)));
     TRY(refresh_events(c));
     c->ckpt_k = k;
     c->ckpt_keep_used = keep;
@@ -2943,10 +2932,8 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
         if (sgi == first_kept) {
             // the trailing segments store their sampled steps right away: the
             // adjoint starts with them and need not replay them
-            for (auto &E : c->evh) {
-                E.hist = E.seg;
-                TRY(upload_slots(c, E, segment_table(E, nt, sgi * k + 1, nt)));
-            }
+            for (auto &E : c->evh) E.hist = E.seg;
+            TRY(upload_slots(c, segment_table(c->evh[0], nt, sgi * k + 1, nt)));
             TRY(refresh_events(c));
         }
         // restart state of every segment (a kept one is replayed only by a
@@ -3031,8 +3018,8 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
                 HIP_TRY(hipMemcpyAsync(E.s, src + 3 * NGB + 9 * NP, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, f->stream));
                 if (nM) HIP_TRY(hipMemcpyAsync(E.M, src + 3 * NGB + 18 * NP, sizeof(float) * nM, hipMemcpyDeviceToDevice, f->stream));
                 E.hist = E.seg;
-                TRY(upload_slots(f, E, segment_table(E, nt, s0, s1)));
             }
+            TRY(upload_slots(f, segment_table(f->evh[0], nt, s0, s1)));
             TRY(refresh_events(f));
             TRY(run_steps(f, s0, s1 - s0 + 1, 1, false));
             HIP_TRY(hipStreamSynchronize(f->stream));
@@ -3040,14 +3027,13 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
         // adjoint steps paired with this segment's forward steps, through the
         // slots of those steps in the segment buffer
         const int a0 = nt + 1 - s1, a1 = nt + 1 - s0;
-        for (int ev = 0; ev < a->n_events; ++ev) {
-            EventHost &A = a->evh[ev];
-            const EventHost &E = f->evh[ev];
-            const std::vector<int> seg_tab = segment_table(E, nt, kept ? first_kept * k + 1 : s0, kept ? nt : s1);
+        {
+            const std::vector<int> seg_tab =
+                segment_table(f->evh[0], nt, kept ? first_kept * k + 1 : s0, kept ? nt : s1);
             std::vector<int> tab(nt + 1, -1);
             for (int it = a0; it <= a1; ++it) tab[it] = seg_tab[nt + 1 - it];
-            A.hist = E.seg;
-            TRY(upload_slots(a, A, tab));
+            for (int ev = 0; ev < a->n_events; ++ev) a->evh[ev].hist = f->evh[ev].seg;
+            TRY(upload_slots(a, tab));
         }
         TRY(refresh_events(a));
         TRY(run_steps(a, a0, a1 - a0 + 1, 2, false));
