@@ -478,6 +478,10 @@ def main():
                     help="only measure the element kernel's HBM bytes per launch with rocprofv3 counter "
                          "passes and write them, stamped with the HIP source's sha256, under profiles/")
     ap.add_argument("--traffic-probe", metavar="WORKLOAD", help=argparse.SUPPRESS)
+    ap.add_argument("--small-gradients", action="store_true",
+                    help="only the small-mesh gradient legs: one event alone and 32 events in one context, "
+                         "on 8x8x4 and 16x16x8 elements")
+    ap.add_argument("--nt", type=int, default=None, help="time steps of the --small-gradients legs")
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per captured step graph in every context (0: direct launches)")
     args = ap.parse_args()
@@ -507,6 +511,16 @@ def main():
 
     if args.inversion:
         print(json.dumps(inversion_leg()))
+        return
+    if args.small_gradients:
+        res = {}
+        for name in ("gradient_8x8x4", "gradient_16x16x8"):
+            print(f"# {name}: one event, then 32 events in one context", file=sys.stderr, flush=True)
+            sp = gradient_leg(k=args.checkpoint_every, workload=name, nt=args.nt)
+            sb = gradient_leg(k=args.checkpoint_every, n_events=32, workload=name, nt=args.nt)
+            sb["per_event_speedup_over_plain"] = sp["s_per_event_gradient"] / sb["s_per_event_gradient"]
+            res[name] = {"plain": sp, "batched": sb}
+        print(json.dumps(res))
         return
 
     from ses3d_amd import configs

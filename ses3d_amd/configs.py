@@ -10,6 +10,7 @@ Workloads (spherical section colat 40-60 deg, lon 10-30 deg, r 5371-6371 km):
                        dt=0.1, nt=4000, 200 receivers
   gradient_48x48x24    anelastic, true = 1D + 2% blobs, 100 receivers, dt=0.15, nt=3000
   gradient_8x8x4       the same physics on 8x8x4 elements (one event per step is launch-bound)
+  gradient_16x16x8     the same on 16x16x8 elements
   batched_64x64x32     32 events, anelastic, 10-400 receivers each, nt=3000
   + EXTRA_SECTIONS     six more anelastic sections of uneven element counts (one
                        event each) advanced with the 32 events in one launch per step
@@ -48,6 +49,7 @@ WORKLOADS = {
     "gradient_48x48x24": Workload("gradient_48x48x24", (48, 48, 24), True, 2, 0.15, 3000, 100, "blobs"),
     # a section small enough that one event's step is launch-bound (batching pays)
     "gradient_8x8x4": Workload("gradient_8x8x4", (8, 8, 4), True, 2, 0.15, 3000, 100, "blobs"),
+    "gradient_16x16x8": Workload("gradient_16x16x8", (16, 16, 8), True, 2, 0.15, 3000, 100, "blobs"),
     "batched_64x64x32": Workload("batched_64x64x32", (64, 64, 32), True, 2, 0.1, 3000, 200, "grf",
                                  n_events=32),
 }
