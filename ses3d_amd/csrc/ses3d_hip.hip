@@ -133,9 +133,8 @@ struct ForceDev {
 
 struct EventDev {
     float *v;          // 3*NGB: (vx, vy) pair plane, then vz at 2*NGB
-    float *d;          // 9*NP  dxux dyuy dzuz dxuy dyux dxuz dzux dyuz dzuy (paired, c9_offset)
-    float *s;          // 9*NP  sxx syy szz sxy syx sxz szx syz szy (split stresses; paired, c9_offset)
-    float *M;          // 18*NP index c*3+k, c: xx yy zz xy xz yz, k: mechanism (paired)
+    float *st;         // NE state blocks of StateLayout<is_diss>::ESB floats: each element's
+                       // split stresses, memory variables and displacement gradient
     float *f;          // 3*NP  element force: (fx, fy) pair plane, then fz at 2*NP;
                        // nodes in fperm order within each element slot
     int src_elem;      // moment-tensor source element or -1
@@ -257,40 +256,56 @@ __host__ __device__ constexpr int fperm(int l, int m, int n) {
          : 124;
 }
 
-// float offset of component c's first point in a 9-component array, and
-// the stride between points (2 in a pair plane, 1 for EZY)
-__host__ __device__ constexpr long long c9_offset(int c, long long NP) {
-    return c == EXX ? 0 : c == EYY ? 1 : c == EZZ ? 2 * NP : c == EXY ? 2 * NP + 1 : c == EXZ ? 4 * NP
-         : c == EYZ ? 4 * NP + 1 : c == EYX ? 6 * NP : c == EZX ? 6 * NP + 1 : 8 * NP;
+// Per-event state: one contiguous block per element holding its 128-node
+// slots of the split stresses, memory variables and displacement gradient,
+// so a workgroup streams its element's state as one run of memory (fewer,
+// longer DRAM bursts than one array per component).  Floats from the block
+// start: stress pair planes (xx,yy) (zz,xy) (xz,yz) first, then the 9 memory
+// variable pair planes (anelastic), which together are all an element outside
+// the absorbing layers touches, then the stress pair (yx,zx) and zy, then the
+// displacement gradient laid out as the stresses.
+struct StateLayout {
+    static constexpr int PS = 2 * TPE;   // floats of one pair plane of one element
+    int m_off, s_out, d_off, esb;
+    __host__ __device__ constexpr StateLayout(int diss)
+        : m_off(3 * PS), s_out(3 * PS + (diss ? 9 * PS : 0)), d_off(s_out + PS + TPE), esb(d_off + 4 * PS + TPE) {}
+};
+// offset in a 9-component (stress or displacement-gradient) group of a block,
+// and the stride between nodes: pairs (xx,yy) (zz,xy) (xz,yz) at 0, PS, 2PS
+// (inner part), (yx,zx) at `outer`, zy at outer + PS
+__host__ __device__ constexpr int c9_block_offset(int c, int outer) {
+    return c == EXX ? 0 : c == EYY ? 1 : c == EZZ ? StateLayout::PS : c == EXY ? StateLayout::PS + 1
+         : c == EXZ ? 2 * StateLayout::PS : c == EYZ ? 2 * StateLayout::PS + 1 : c == EYX ? outer
+         : c == EZX ? outer + 1 : outer + StateLayout::PS;
 }
 __host__ __device__ constexpr int c9_stride(int c) { return c == EZY ? 1 : 2; }
-// inner: the six components every element touches (planes 0..2)
+// inner: the six components every element touches (three pair planes from b)
 template <int NT>
-__device__ __forceinline__ void ld9_inner(const float *b, size_t NP, unsigned pb2, float *x) {
-    f2v t = ldo2<NT>(b, pb2);
+__device__ __forceinline__ void ld9_inner(const float *b, unsigned o2, float *x) {
+    f2v t = ldo2<NT>(b, o2);
     x[EXX] = t.x; x[EYY] = t.y;
-    t = ldo2<NT>(b + 2 * NP, pb2);
+    t = ldo2<NT>(b + StateLayout::PS, o2);
     x[EZZ] = t.x; x[EXY] = t.y;
-    t = ldo2<NT>(b + 4 * NP, pb2);
+    t = ldo2<NT>(b + 2 * StateLayout::PS, o2);
     x[EXZ] = t.x; x[EYZ] = t.y;
 }
-// outer: the second members of the split pairs (plane 3 and EZY)
+// outer: the second members of the split pairs (pair plane at b, EZY after it)
 template <int NT>
-__device__ __forceinline__ void ld9_outer(const float *b, size_t NP, unsigned pb2, unsigned pb, float *x) {
-    const f2v t = ldo2<NT>(b + 6 * NP, pb2);
+__device__ __forceinline__ void ld9_outer(const float *b, unsigned o2, unsigned o, float *x) {
+    const f2v t = ldo2<NT>(b, o2);
     x[EYX] = t.x; x[EZX] = t.y;
-    x[EZY] = ldo<NT>(b + 8 * NP, pb);
+    x[EZY] = ldo<NT>(b + StateLayout::PS, o);
 }
 template <int NT>
-__device__ __forceinline__ void st9_inner(float *b, size_t NP, unsigned pb2, const float *x) {
-    sto2<NT>(b, pb2, x[EXX], x[EYY]);
-    sto2<NT>(b + 2 * NP, pb2, x[EZZ], x[EXY]);
-    sto2<NT>(b + 4 * NP, pb2, x[EXZ], x[EYZ]);
+__device__ __forceinline__ void st9_inner(float *b, unsigned o2, const float *x) {
+    sto2<NT>(b, o2, x[EXX], x[EYY]);
+    sto2<NT>(b + StateLayout::PS, o2, x[EZZ], x[EXY]);
+    sto2<NT>(b + 2 * StateLayout::PS, o2, x[EXZ], x[EYZ]);
 }
 template <int NT>
-__device__ __forceinline__ void st9_outer(float *b, size_t NP, unsigned pb2, unsigned pb, const float *x) {
-    sto2<NT>(b + 6 * NP, pb2, x[EYX], x[EZX]);
-    sto<NT>(b + 8 * NP, pb, x[EZY]);
+__device__ __forceinline__ void st9_outer(float *b, unsigned o2, unsigned o, const float *x) {
+    sto2<NT>(b, o2, x[EYX], x[EZX]);
+    sto<NT>(b + StateLayout::PS, o, x[EZY]);
 }
 
 __device__ __forceinline__ void stnt(float *p, float v) {
@@ -414,6 +429,10 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
     // element in an absorbing layer? (uniform over the element's wavefronts)
     const bool pml = act && (ldg(Ms.PMLX + ei) | ldg(Ms.PMLY + ej) | ldg(Ms.PMLZ + ek));
     const bool keep_d = (MODE != 0) || pml;
+    // the element's state block (uniform base) and the lane's offsets in its planes
+    constexpr StateLayout SL(DISS);
+    float *const sb = E.st + (size_t)(act ? e : 0) * SL.esb;
+    const unsigned o = 4u * (unsigned)nd, o2 = 2u * o;
 
     // All loads of the step are issued before the first barrier.
     float vx = 0.f, vy = 0.f, vz = 0.f;
@@ -427,15 +446,15 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
         vy = vxy.y;
         vz = ldnt(E.v + 2 * Ms.NGB + g);
         if (keep_d) {
-            ld9_inner<NT>(E.d, NP, pb2, d9);
-            ld9_outer<NT>(E.d, NP, pb2, pb, d9);
+            ld9_inner<NT>(sb + SL.d_off, o2, d9);
+            ld9_outer<NT>(sb + SL.d_off + 3 * StateLayout::PS, o2, o, d9);
         }
-        ld9_inner<NT>(E.s, NP, pb2, s9);
-        if (pml) ld9_outer<NT>(E.s, NP, pb2, pb, s9);
+        ld9_inner<NT>(sb, o2, s9);
+        if (pml) ld9_outer<NT>(sb + SL.s_out, o2, o, s9);
         if (DISS) {
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
-                const f2v t = ldo2<NT>(E.M + 2 * q * NP, pb2);
+                const f2v t = ldo2<NT>(sb + SL.m_off + q * StateLayout::PS, o2);
                 Mv[2 * q] = t.x;
                 Mv[2 * q + 1] = t.y;
             }
@@ -511,12 +530,12 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             dyuz = d9[EYZ] + dt * eyz;
             dxuy = d9[EXY] + dt * exy;
             dyux = d9[EYX] + dt * eyx;
-            float *d = E.d;
-            sto2<NT>(d, pb2, dxux, dyuy);
-            sto2<NT>(d + 2 * NP, pb2, dzuz, dxuy);
-            sto2<NT>(d + 4 * NP, pb2, dxuz, dyuz);
-            sto2<NT>(d + 6 * NP, pb2, dyux, dzux);
-            sto<NT>(d + 8 * NP, pb, dzuy);
+            float *d = sb + SL.d_off;
+            sto2<NT>(d, o2, dxux, dyuy);
+            sto2<NT>(d + StateLayout::PS, o2, dzuz, dxuy);
+            sto2<NT>(d + 2 * StateLayout::PS, o2, dxuz, dyuz);
+            sto2<NT>(d + 3 * StateLayout::PS, o2, dyux, dzux);
+            sto<NT>(d + 4 * StateLayout::PS, o, dzuy);
         }
         // absorbing-layer terms (:123-134); exact no-ops outside the layers
         float px = 0.f, py = 0.f, pz = 0.f, taper = 1.f;
@@ -581,7 +600,8 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
                     const float q = Ms.fast_tau ? div_recip(dM, Ms.tau_p[k], Ms.rtau_p[k]) : dM / Ms.tau_p[k];
                     const float Mn = (Mo - q) - Ms.Lk[k] * ev6[c];
                     Mv[c * 3 + k] = Mn;
-                    if ((c * 3 + k) & 1) sto2<NT>(E.M + (c * 3 + k - 1) * NP, pb2, Mv[c * 3 + k - 1], Mn);
+                    if ((c * 3 + k) & 1)
+                        sto2<NT>(sb + SL.m_off + ((c * 3 + k) >> 1) * StateLayout::PS, o2, Mv[c * 3 + k - 1], Mn);
                 }
         } else {
             const float L = lam * ((exx + eyy) + ezz);
@@ -602,7 +622,6 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
             }
         }
         // split-stress integration (:208-219)
-        float *s = E.s;
         if (pml) {
             s9[EXX] = s9[EXX] + dt * (sxx - px * s9[EXX]);
             s9[EYY] = s9[EYY] + dt * (syy - py * s9[EYY]);
@@ -643,10 +662,10 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
         if (pml) {
 #pragma unroll
             for (int c = 0; c < 9; ++c) s9[c] = s9[c] * taper;
-            st9_inner<NT>(s, NP, pb2, s9);
-            st9_outer<NT>(s, NP, pb2, pb, s9);
+            st9_inner<NT>(sb, o2, s9);
+            st9_outer<NT>(sb + SL.s_out, o2, o, s9);
         } else {
-            st9_inner<NT>(s, NP, pb2, s9);
+            st9_inner<NT>(sb, o2, s9);
         }
 
         if (MODE == 1 && hcomp) {
@@ -1018,22 +1037,23 @@ __global__ void k_verify_div(const float *b, const float *y, int nd, unsigned lo
 
 // One component of an element-major array (point p at src[p * stride]) into
 // the reference layout (element fastest, node slowest): dst[e + NE * nd].
+// element e's node nd at src[e * es + nd * stride] (es: the element stride)
 template <typename T>
-__global__ void k_to_ref(T *__restrict__ dst, const T *__restrict__ src, long long NE, int stride) {
+__global__ void k_to_ref(T *__restrict__ dst, const T *__restrict__ src, long long NE, int stride, long long es) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= NE * 125) return;
     const long long nd = i / NE, e = i - nd * NE;
-    dst[i] = src[(e * EST + nd) * stride];
+    dst[i] = src[e * es + nd * stride];
 }
 
 // Inverse of k_to_ref: reference layout into one component of an
 // element-major array (the 3 padding values of each slot are left as they are).
 template <typename T>
-__global__ void k_from_ref(T *__restrict__ dst, const T *__restrict__ src, long long NE, int stride) {
+__global__ void k_from_ref(T *__restrict__ dst, const T *__restrict__ src, long long NE, int stride, long long es) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= NE * 125) return;
     const long long nd = i / NE, e = i - nd * NE;
-    dst[(e * EST + nd) * stride] = src[i];
+    dst[e * es + nd * stride] = src[i];
 }
 
 // float32 accumulators widened into a float64 reference-layout array
@@ -1228,7 +1248,7 @@ struct EventHost {
     ses3d::SourceLoc sloc;
     std::vector<ses3d::PointLoc> recs, ads;
     // device
-    float *v = nullptr, *d = nullptr, *s = nullptr, *M = nullptr, *f = nullptr;
+    float *v = nullptr, *st = nullptr, *f = nullptr;   // st: NE state blocks (StateLayout)
     float *src_md = nullptr, *so_d = nullptr;
     float *seis = nullptr;     // nr x 3 x nt
     float *rec_w = nullptr;    // nr x 125
@@ -1412,11 +1432,12 @@ int download(ses3d_hip_context *c, void *dst, const void *src, size_t bytes) {
 // values between points) to a host array in reference layout, transposed on
 // the GPU into a scratch buffer kept by the context.
 template <typename T>
-int download_ref(ses3d_hip_context *c, T *host, const T *comp, int stride) {
+int download_ref(ses3d_hip_context *c, T *host, const T *comp, int stride, long long es = -1) {
     const size_t n = (size_t)c->NPTS;
     TRY(dfit(c, &c->conv, c->conv_cap, (n * sizeof(T) + sizeof(float) - 1) / sizeof(float)));
     T *d = (T *)c->conv;
-    hipLaunchKernelGGL(k_to_ref<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d, comp, c->NE, stride);
+    hipLaunchKernelGGL(k_to_ref<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d, comp, c->NE, stride,
+                       es < 0 ? (long long)EST * stride : es);
     HIP_TRY(hipGetLastError());
     return download(c, host, d, sizeof(T) * n);
 }
@@ -1429,7 +1450,8 @@ int upload_ref(ses3d_hip_context *c, T *comp, const T *host, int stride) {
     TRY(dfit(c, &c->conv, c->conv_cap, (n * sizeof(T) + sizeof(float) - 1) / sizeof(float)));
     T *d = (T *)c->conv;
     TRY(upload(c, d, host, sizeof(T) * n));
-    hipLaunchKernelGGL(k_from_ref<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, comp, d, c->NE, stride);
+    hipLaunchKernelGGL(k_from_ref<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, comp, d, c->NE, stride,
+                       (long long)EST * stride);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1441,6 +1463,8 @@ int zero(ses3d_hip_context *c, void *p, size_t bytes) {
 }
 
 int elem_blocks(const ses3d_hip_context *c, int epb) { return (int)((c->NE + epb - 1) / epb); }
+// floats of one event's state blocks
+size_t state_floats(const ses3d_hip_context *c) { return (size_t)c->NE * StateLayout(c->cfg.is_diss).esb; }
 long long node_blocks(const ses3d_hip_context *c) { return (long long)c->NBX * c->NBY * c->NBZ; }
 // Bytes of element-local state one step streams (all events): displacement
 // gradient, split stresses, memory variables, element force, coefficients.
@@ -1857,7 +1881,7 @@ int refresh_events(ses3d_hip_context *c) {
     for (int ev = 0; ev < c->n_events; ++ev) {
         EventHost &E = c->evh[ev];
         EventDev &D = h[ev];
-        D.v = E.v; D.d = E.d; D.s = E.s; D.M = E.M; D.f = E.f;
+        D.v = E.v; D.st = E.st; D.f = E.f;
         D.src_elem = -1;
         if (E.have_source && E.source_type == 10 && E.sloc.is_source &&
             (c->cfg.adjoint_flag == 0 || c->cfg.adjoint_flag == 1))
@@ -2068,7 +2092,7 @@ std::vector<int> segment_table(const EventHost &E, int nt, int s0, int s1) {
 int ensure_checkpoints(ses3d_hip_context *c, int k, int keep) {
     const int nseg = (c->cfg.nt + k - 1) / k;
     const size_t NP = (size_t)c->NP, NGB = (size_t)c->NGB;
-    const size_t ck = 3 * NGB + 18 * NP + (c->cfg.is_diss ? 18 * NP : 0);
+    const size_t ck = 3 * NGB + state_floats(c);
     for (auto &E : c->evh)
         if ((int)E.ckpt.size() != nseg) {
             HIP_TRY(hipStreamSynchronize(c->stream));
@@ -2251,9 +2275,7 @@ int ses3d_hip_create(const ses3d_hip_config *cfg, int32_t n_events, ses3d_hip_co
     for (int ev = 0; ev < n_events && !rc; ++ev) {
         EventHost &E = c->evh[ev];
         rc = rc ? rc : dalloc(c, &E.v, 3 * NGB);
-        rc = rc ? rc : dalloc(c, &E.d, 9 * NP);
-        rc = rc ? rc : dalloc(c, &E.s, 9 * NP);
-        if (cfg->is_diss) rc = rc ? rc : dalloc(c, &E.M, 18 * NP);
+        rc = rc ? rc : dalloc(c, &E.st, state_floats(c));
         rc = rc ? rc : dalloc(c, &E.f, 3 * NP);
         rc = rc ? rc : dalloc(c, &E.src_md, 9 * 125);
         rc = rc ? rc : dalloc(c, &E.so_d, cfg->nt);
@@ -2446,9 +2468,7 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     for (int ev = 0; ev < c->n_events; ++ev) {
         EventHost &E = c->evh[ev];
         TRY(zero(c, E.v, sizeof(float) * 3 * c->NGB));
-        TRY(zero(c, E.d, sizeof(float) * 9 * c->NP));
-        TRY(zero(c, E.s, sizeof(float) * 9 * c->NP));
-        if (E.M) TRY(zero(c, E.M, sizeof(float) * 18 * c->NP));
+        TRY(zero(c, E.st, sizeof(float) * state_floats(c)));
         if (E.grad) TRY(zero(c, E.grad, sizeof(double) * 5 * c->NP));
         TRY(upload(c, E.so_d, E.so.data(), sizeof(float) * nt));
         std::vector<float> md(9 * 125, 0.f);
@@ -2645,7 +2665,7 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
     if (!name || !out) return fail(SES3D_HIP_EINVAL, "null argument");
     EventHost &E = c->evh[ev];
     const std::string nm(name);
-    const long long NP = c->NP;
+    const StateLayout SL(c->cfg.is_diss);
     static const char *dn[9] = {"dxux", "dyuy", "dzuz", "dxuy", "dyux", "dxuz", "dzux", "dyuz", "dzuy"};
     static const char *sn[9] = {"sxx", "syy", "szz", "sxy", "syx", "sxz", "szx", "syz", "szy"};
     static const char *mn[6] = {"Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz"};
@@ -2657,13 +2677,15 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
             if (c->cfg.adjoint_flag == 0)
                 return fail(SES3D_HIP_EINVAL, "with adjoint_flag=0 the displacement gradient is integrated only "
                                               "inside the absorbing layers (the only place it acts); use adjoint_flag 1 or 2");
-            return download_ref(c, out, E.d + c9_offset(k, NP), c9_stride(k));
+            return download_ref(c, out, E.st + SL.d_off + c9_block_offset(k, 3 * StateLayout::PS), c9_stride(k),
+                                SL.esb);
         }
         if (nm == sn[k]) {
-            TRY(download_ref(c, out, E.s + c9_offset(k, NP), c9_stride(k)));
+            TRY(download_ref(c, out, E.st + c9_block_offset(k, SL.s_out), c9_stride(k), SL.esb));
             if (pair[k] != k) {
                 std::vector<float> first(c->NPTS);
-                TRY(download_ref(c, first.data(), E.s + c9_offset(pair[k], NP), c9_stride(pair[k])));
+                TRY(download_ref(c, first.data(), E.st + c9_block_offset(pair[k], SL.s_out), c9_stride(pair[k]),
+                                 SL.esb));
                 std::vector<char> inner(c->NE);
                 for (long long e = 0; e < c->NE; ++e) {
                     const int i = (int)(e % c->NX), j = (int)((e / c->NX) % c->NY), kk = (int)(e / ((long long)c->NX * c->NY));
@@ -2677,10 +2699,11 @@ int ses3d_hip_get_field(ses3d_hip_context *c, int32_t ev, const char *name, floa
     }
     for (int k = 0; k < 6; ++k)
         if (nm == mn[k]) {
-            if (!E.M) return fail(SES3D_HIP_EINVAL, "memory variables exist only when is_diss=1");
+            if (!c->cfg.is_diss) return fail(SES3D_HIP_EINVAL, "memory variables exist only when is_diss=1");
             for (int mech = 0; mech < 3; ++mech) {
                 const int i = k * 3 + mech;   // pair plane i / 2, member i % 2
-                TRY(download_ref(c, out + mech * c->NPTS, E.M + (long long)(i & ~1) * NP + (i & 1), 2));
+                TRY(download_ref(c, out + mech * c->NPTS, E.st + SL.m_off + (i >> 1) * StateLayout::PS + (i & 1), 2,
+                                 SL.esb));
             }
             return 0;
         }
@@ -2771,9 +2794,10 @@ int ses3d_hip_set_model_device(ses3d_hip_context *c, const float *rhoinv, const 
     const float *src[5] = {rhoinv, A, B, C, Q};
     float *dst[5] = {c->rhoinv, c->A, c->B, c->C, c->Q};
     for (int f = 0; f < 5; ++f)
-        if (dst[f] && src[f]) hipLaunchKernelGGL(k_from_ref<float>, grid, block, 0, c->stream, dst[f], src[f], c->NE, 1);
-    hipLaunchKernelGGL(k_from_ref<float>, grid, block, 0, c->stream, c->lm, lambda, c->NE, 2);   // (lambda, mu) pairs
-    hipLaunchKernelGGL(k_from_ref<float>, grid, block, 0, c->stream, c->lm + 1, mu, c->NE, 2);
+        if (dst[f] && src[f])
+            hipLaunchKernelGGL(k_from_ref<float>, grid, block, 0, c->stream, dst[f], src[f], c->NE, 1, (long long)EST);
+    hipLaunchKernelGGL(k_from_ref<float>, grid, block, 0, c->stream, c->lm, lambda, c->NE, 2, 2LL * EST);   // (lambda, mu) pairs
+    hipLaunchKernelGGL(k_from_ref<float>, grid, block, 0, c->stream, c->lm + 1, mu, c->NE, 2, 2LL * EST);
     HIP_TRY(hipGetLastError());
     // anisotropy test on the device (A, B, C all zero: isotropic kernels)
     if (!c->flag_d) TRY(dalloc(c, &c->flag_d, 1));
@@ -2809,7 +2833,7 @@ int ses3d_hip_get_kernels_device(ses3d_hip_context *c, int32_t ev, double *g_rho
                                stride);
         else
             hipLaunchKernelGGL(k_to_ref<double>, grid, block, 0, c->stream, outs[k], (const double *)E.grad + off,
-                               c->NE, stride);
+                               c->NE, stride, (long long)EST * stride);
     }
     if (g_Q) hipLaunchKernelGGL(k_q_normalise, grid, block, 0, c->stream, g_Q, c->lm, c->Q, c->NE, f32 ? 1 : 0);
     HIP_TRY(hipGetLastError());
@@ -2919,7 +2943,7 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
     const int keep = std::min(c->ckpt_keep, nseg);
     const int first_kept = nseg - keep;   // the first segment whose sampled steps are kept
     const size_t NP = (size_t)c->NP, NGB = (size_t)c->NGB;
-    const size_t nM = c->cfg.is_diss ? 18 * NP : 0;
+    const size_t nst = state_floats(c);
     TRY(ensure_checkpoints(c, k, keep));
     // the forward pass keeps the displacement gradient (mode 1) but stores no sampled step
     for (auto &E : c->evh) E.hist = nullptr;
@@ -2941,9 +2965,7 @@ int ses3d_hip_forward_checkpointed(ses3d_hip_context *c, int32_t k) {
         for (auto &E : c->evh) {
             float *dst = E.ckpt[sgi];
             HIP_TRY(hipMemcpyAsync(dst, E.v, sizeof(float) * 3 * NGB, hipMemcpyDeviceToDevice, c->stream));
-            HIP_TRY(hipMemcpyAsync(dst + 3 * NGB, E.d, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, c->stream));
-            HIP_TRY(hipMemcpyAsync(dst + 3 * NGB + 9 * NP, E.s, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, c->stream));
-            if (nM) HIP_TRY(hipMemcpyAsync(dst + 3 * NGB + 18 * NP, E.M, sizeof(float) * nM, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(dst + 3 * NGB, E.st, sizeof(float) * nst, hipMemcpyDeviceToDevice, c->stream));
         }
         const int s0 = sgi * k + 1, s1 = std::min((sgi + 1) * k, nt);
         TRY(run_steps(c, s0, s1 - s0 + 1, 1, true));
@@ -2973,7 +2995,7 @@ int ses3d_hip_prepare_checkpoints(ses3d_hip_context *c, int32_t k) {
     c->last_seg_ready = false;
     // first touch of every page here rather than in the forward pass
     const size_t NP = (size_t)c->NP, NGB = (size_t)c->NGB;
-    const size_t ck = 3 * NGB + 18 * NP + (c->cfg.is_diss ? 18 * NP : 0);
+    const size_t ck = 3 * NGB + state_floats(c);
     for (auto &E : c->evh) {
         for (float *q : E.ckpt) HIP_TRY(hipMemsetAsync(q, 0, sizeof(float) * ck, c->stream));
         if (E.seg && E.seg_slots > 0)
@@ -2999,7 +3021,7 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
     const int nseg = (nt + k - 1) / k;
     const int first_kept = nseg - f->ckpt_keep_used;
     const size_t NP = (size_t)f->NP, NGB = (size_t)f->NGB;
-    const size_t nM = f->cfg.is_diss ? 18 * NP : 0;
+    const size_t nst = state_floats(f);
     TRY(ensure_segment_buffer(f, k, f->ckpt_keep_used));
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
         const int s0 = sgi * k + 1, s1 = std::min((sgi + 1) * k, nt);
@@ -3014,9 +3036,7 @@ int ses3d_hip_adjoint_checkpointed(ses3d_hip_context *a, ses3d_hip_context *f) {
                 EventHost &E = f->evh[ev];
                 const float *src = E.ckpt[sgi];
                 HIP_TRY(hipMemcpyAsync(E.v, src, sizeof(float) * 3 * NGB, hipMemcpyDeviceToDevice, f->stream));
-                HIP_TRY(hipMemcpyAsync(E.d, src + 3 * NGB, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, f->stream));
-                HIP_TRY(hipMemcpyAsync(E.s, src + 3 * NGB + 9 * NP, sizeof(float) * 9 * NP, hipMemcpyDeviceToDevice, f->stream));
-                if (nM) HIP_TRY(hipMemcpyAsync(E.M, src + 3 * NGB + 18 * NP, sizeof(float) * nM, hipMemcpyDeviceToDevice, f->stream));
+                HIP_TRY(hipMemcpyAsync(E.st, src + 3 * NGB, sizeof(float) * nst, hipMemcpyDeviceToDevice, f->stream));
                 E.hist = E.seg;
             }
             TRY(upload_slots(f, segment_table(f->evh[0], nt, s0, s1)));
