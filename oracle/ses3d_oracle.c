@@ -34,9 +34,10 @@
  *                     (ses3d_grad.f90:25-85) and the shear-Q kernel of
  *                     ses3d_grad_Q.f90:111-128,151-153.
  *
- * Parity pins for each function are listed in the tests that use it; the
- * shear-Q kernel has no reference output (its writer is commented out at
- * ses3d_output.f90:346-370) and is "parity unpinned" against the reference.
+ * Parity pins for each function are listed in the tests that use it.  The
+ * reference never writes the shear-Q kernel (ses3d_output.f90:346-370 is
+ * commented out); it is pinned against the reference's ses3d_grad_Q.f90
+ * built with oracle/ref_grad_q_harness.f90, which writes it.
  */
 #include <math.h>
 #include <stdint.h>

@@ -7,6 +7,8 @@
 //                                            the element force in node-block order, fperm)
 //                         components stored in pairs (two floats per point side by side) so a
 //                         lane moves two components with one 8-byte access
+//   per-event state       one contiguous block per element (StateLayout): split stresses,
+//                         memory variables and displacement gradient in pair planes
 //   global fields         v[blocked]         4x4x4 node blocks: node (gx,gy,gz) lives in
 //                                            block (gx>>2, gy>>2, gz>>2) at (gz&3)*16+(gy&3)*4+(gx&3);
 //                                            (vx, vy) as one pair plane, vz alone
@@ -16,10 +18,10 @@
 //   k_element    gather v -> 9 velocity gradients (5-term GLL contractions through LDS)
 //                -> spherical terms -> displacement-gradient update -> absorbing-layer terms
 //                -> stress rates (elastic or 3-mechanism anelastic) -> split-stress update
-//                -> weak-form divergence (LDS) + moment source -> element force;
-//                memory-variable update, stress taper, forward-field store (optionally at a
-//                lower polynomial degree) and (adjoint) kernel accumulation fused
-//   k_forces     point forces / adjoint sources added to the element force
+//                -> weak-form divergence (LDS) + moment source -> point forces / adjoint
+//                sources -> element force; memory-variable update, stress taper,
+//                forward-field store (optionally at a lower polynomial degree) and (adjoint)
+//                kernel accumulation fused
 //   k_node       gather the element forces of the <=8 elements sharing a node in the
 //                reference's assembly order, divide by the mass, velocity update, taper
 // All float32 arithmetic follows the reference's left-to-right order and is compiled with
