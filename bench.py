@@ -478,6 +478,9 @@ def main():
                     help="only measure the element kernel's HBM bytes per launch with rocprofv3 counter "
                          "passes and write them, stamped with the HIP source's sha256, under profiles/")
     ap.add_argument("--traffic-probe", metavar="WORKLOAD", help=argparse.SUPPRESS)
+    ap.add_argument("--gradient-probe", type=int, metavar="NT",
+                    help="only one event-gradient of the 48x48x24 gradient workload at nt=NT with checkpoints "
+                         "every --checkpoint-every steps (short runs for counter passes)")
     ap.add_argument("--small-gradients", action="store_true",
                     help="only the small-mesh gradient legs: one event alone and 32 events in one context, "
                          "on 8x8x4 and 16x16x8 elements")
@@ -511,6 +514,9 @@ def main():
 
     if args.inversion:
         print(json.dumps(inversion_leg()))
+        return
+    if args.gradient_probe:
+        print(json.dumps(gradient_leg(k=args.checkpoint_every, nt=args.gradient_probe)))
         return
     if args.small_gradients:
         res = {}
