@@ -12,8 +12,17 @@ What the tolerances rest on:
 * in anelastic models the reference's kernel formulas are the elastic ones
   (mu instead of the unrelaxed mu_tau = (1 + sum D_p / Q) mu), a bias of order
   sum D_p / Q ~ 2% for the shear terms; its shear-Q kernel is research code
-  (ses3d_grad_Q.f90, off-diagonal terms not doubled), only its sign and order
-  of magnitude are checked.
+  (ses3d_grad_Q.f90, off-diagonal terms not doubled, the unrelaxed-modulus
+  term left out) and is pinned bit for bit to the reference's own build
+  elsewhere; here its mapping to dchi/dQ (the kernel is for relative
+  perturbations, so it is divided by Q and weighted like the elastic ones) is
+  checked for sign and size: the measured distance from the finite difference
+  is 0.33, and an unscaled kernel (no division by Q ~ 100-600) or a squared Q
+  would miss by orders of magnitude.
+The runs are deterministic (fixed seeds, bit-reproducible kernels), so the
+bounds sit a little above the measured distances: density 0.19 / 0.093 at
+dt = 0.25 / 0.125, vp, vsh, vsv 0.0016-0.0022, all four 0.010 / 0.0042
+(elastic); vsh 0.018, all 0.029 (anelastic).
 """
 import numpy as np
 import pytest
@@ -73,20 +82,20 @@ def test_elastic_gradient_vs_finite_differences():
     e1 = _fd(0.25, False)
     e2 = _fd(0.125, False)
     for p in ("vp", "vsh", "vsv"):
-        assert e1[p][2] <= 0.005, (p, e1[p])
-    assert e1["all"][2] <= 0.02, e1["all"]
+        assert e1[p][2] <= 0.003, (p, e1[p])
+    assert e1["all"][2] <= 0.013, e1["all"]
     assert e2["all"][2] <= e1["all"][2], (e1["all"], e2["all"])
-    assert e1["rho"][2] <= 0.3, e1["rho"]
+    assert e1["rho"][2] <= 0.22, e1["rho"]
     assert e2["rho"][2] <= 0.6 * e1["rho"][2], (e1["rho"], e2["rho"])   # first order in dt
 
 
 def test_anelastic_gradient_vs_finite_differences():
     e = _fd(0.25, True)
     for p in ("vp", "vsv"):
-        assert e[p][2] <= 0.005, (p, e[p])
-    assert e["vsh"][2] <= 0.04, e["vsh"]
-    assert e["all"][2] <= 0.05, e["all"]
-    assert np.sign(e["Q"][0]) == np.sign(e["Q"][1]) and e["Q"][2] <= 0.5, e["Q"]
+        assert e[p][2] <= 0.003, (p, e[p])
+    assert e["vsh"][2] <= 0.025, e["vsh"]
+    assert e["all"][2] <= 0.035, e["all"]
+    assert np.sign(e["Q"][0]) == np.sign(e["Q"][1]) and e["Q"][2] <= 0.38, e["Q"]
 
 
 def test_steepest_descent_update_lowers_misfit():
