@@ -405,7 +405,11 @@ def inversion_leg(n_events: int = 16, nt: int = 4000, k: int = 100, step_frac: f
     accumulation, one event per solve: the checkpoints of one event take ~160 GB),
     a steepest-descent update of vp, vsh, vsv (step_frac of their means along the
     max-normalised gradient), and the misfit before and after it, measured by plain
-    single-event forward solves."""
+    single-event forward solves.  The same gradient is also timed without
+    autograd (Problem.misfit_and_gradient: each batch's checkpointed forward and
+    adjoint back to back), where no forward solve is repeated; autograd's
+    forward keeps the checkpoints of the last batch only, so its backward
+    recomputes the checkpointed forward of every other batch."""
     import torch
     from dataclasses import replace
     from ses3d_amd import configs
@@ -438,6 +442,17 @@ def inversion_leg(n_events: int = 16, nt: int = 4000, k: int = 100, step_frac: f
     chi0.backward()
     t_grad = time.perf_counter() - t1
     g = [t.grad.numpy() for t in x]
+    print(f"# autograd gradient of {n_events} events in {t_grad:.1f} s", file=sys.stderr, flush=True)
+    # the same gradient without autograd: checkpointed forward and adjoint of each
+    # batch back to back (no recomputed forward for the batches whose checkpoints
+    # autograd's forward could not keep)
+    t1 = time.perf_counter()
+    model0 = prob.fields(rho0, vp0, vs0, vs0)
+    chi_e, K = prob.misfit_and_gradient(model0)
+    ge = prob.gradient_from_kernels(K, model0)
+    t_eager = time.perf_counter() - t1
+    same = chi_e == float(chi0.detach()) and all(np.array_equal(ge[n], g[i])
+                                                 for i, n in enumerate(("rho", "vp", "vsh", "vsv")))
     scale = [0.0, step_frac * vp0.mean(), step_frac * vs0.mean(), step_frac * vs0.mean()]
     new = [x[i].detach().numpy() - scale[i] * g[i] / max(np.abs(g[i]).max(), 1e-300) for i in range(4)]
     t2 = time.perf_counter()
@@ -447,6 +462,7 @@ def inversion_leg(n_events: int = 16, nt: int = 4000, k: int = 100, step_frac: f
     return {"workload": f"inversion iteration, {w.name}, {n_events} events, nt={nt}, dt={w.dt}, checkpoints every {k}",
             "events": n_events, "gll_points": w.n_points,
             "s_per_event_gradient": t_grad / n_events, "s_gradient_total": t_grad, "s_data": t_data,
+            "s_per_event_gradient_direct": t_eager / n_events, "direct_equals_autograd": bool(same),
             "s_judge_two_misfits": t_judge, "misfit_autograd": float(chi0.detach()),
             "misfit_before": chi_before, "misfit_after": chi_after, "misfit_ratio": chi_after / chi_before}
 

@@ -62,12 +62,12 @@ int fail(int code, const std::string &msg) {
 #define SES3D_EPB 1
 #endif
 // minimum waves per SIMD the element kernels are compiled for: 5 fits them in
-// 96 VGPRs without spilling, except the field-storing kernel with 2 elements
-// per workgroup, which is compiled for 4
+// 96 VGPRs without spilling, except the field-storing kernels (mode 1), which
+// are compiled for 4 (at 5 they spill two registers and run 0.5-0.8% slower)
 #ifndef SES3D_WAVES_PER_EU
 #define SES3D_WAVES_PER_EU 5
 #endif
-#define WAVES_MIN(MODE, EPB) ((MODE) == 1 && (EPB) == 2 ? 4 : (SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1))
+#define WAVES_MIN(MODE, EPB) ((MODE) == 1 ? 4 : (SES3D_WAVES_PER_EU ? SES3D_WAVES_PER_EU : 1))
 // Launch schedule: elements per element-kernel workgroup (1, 2 or 4; 128
 // lanes each, 125 active), lanes per node-kernel workgroup (128, 256 or 512:
 // 2, 4 or 8 node blocks of 64) and the XCD-aware workgroup order.  Every
