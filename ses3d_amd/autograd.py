@@ -208,8 +208,9 @@ class Problem:
             return out
 
         # the held batch first (its checkpoints are overwritten by any other
-        # batch of its size), added last, so the sum runs in event order
-        held = adjoint(st.held, recompute=False)
+        # batch of its size), added in its place, so the sum runs in event
+        # order; once released (a second backward), every batch is recomputed
+        held = adjoint(st.held, recompute=False) if st.held is not None else None
         K = None
         for bi in range(len(batches)):
             for kk in (held if bi == st.held else adjoint(bi, recompute=True)):
@@ -329,10 +330,12 @@ class _GradState:
         return self.ctx[n]
 
     def close(self):
+        """Free the contexts (and the held batch's checkpoints with them)."""
         for f, a in self.ctx.values():
             f.close()
             a.close()
         self.ctx = {}
+        self.held = None
 
     def __del__(self):
         try:
@@ -373,7 +376,13 @@ try:
         @staticmethod
         def backward(ctx, grad_out):
             problem, st = ctx.problem, ctx.state
-            K = problem.gradient_kernels(st)
+            try:
+                K = problem.gradient_kernels(st)
+            finally:
+                # the autograd node outlives backward as long as the misfit
+                # tensor does: its contexts and checkpoint storage are freed
+                # here (a second backward recomputes every batch)
+                st.close()
             grads = problem.gradient_from_kernels(K, st.model)
             out = []
             for name, meta in zip(("rho", "vp", "vsh", "vsv", "Q"), ctx.meta):

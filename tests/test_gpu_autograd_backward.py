@@ -67,6 +67,35 @@ def test_adjoint_runs_in_backward_only(monkeypatch, per_batch):
         np.testing.assert_array_equal(t.grad.numpy(), g_ref[name], err_msg=name)
 
 
+def test_backward_frees_its_contexts_and_a_second_backward_recomputes(monkeypatch):
+    """The autograd node lives as long as the misfit tensor: backward must free
+    the contexts (and the checkpoints they hold) itself; a second backward
+    (retain_graph) recomputes every batch and gives the same gradient."""
+    from ses3d_amd.autograd import misfit
+    from ses3d_amd.solver import Solver
+    prob, base = _problem(1)
+    made = []
+    orig = Solver.from_run.__func__
+
+    def tracked(cls, *a, **k):
+        s = orig(cls, *a, **k)
+        made.append(s)
+        return s
+    monkeypatch.setattr(Solver, "from_run", classmethod(tracked))
+    x = [torch.tensor(a, requires_grad=True) for a in base]
+    chi = misfit(prob, *x)
+    chi.backward(retain_graph=True)
+    assert made and all(s.h is None for s in made), "contexts left open after backward"
+    g1 = [t.grad.clone() for t in x]
+    for t in x:
+        t.grad = None
+    n0 = len(made)
+    chi.backward()
+    assert len(made) > n0 and all(s.h is None for s in made)
+    for name, a, t in zip(("rho", "vp", "vsh", "vsv", "Q"), g1, x):
+        np.testing.assert_array_equal(t.grad.numpy(), a.numpy(), err_msg=name)
+
+
 def test_forward_without_grad_runs_no_adjoint(monkeypatch):
     from ses3d_amd.autograd import misfit
     prob, base = _problem(2)
