@@ -650,14 +650,15 @@ def main():
         gd = gradient_leg(k=args.checkpoint_every, history_degree=2, kernel_bits=32, keep=nseg)
         gd["speedup_over_float64"] = out["gradient"]["s_per_event_gradient"] / gd["s_per_event_gradient"]
         out["gradient_f32_deg2_keepall"] = gd
-        # where one event's step is launch-bound, batching pays: 32 events on 8x8x4 in one
-        # context against one event alone
-        print("# small-mesh gradient legs, plain and 32 events batched", file=sys.stderr, flush=True)
+        # where one event's step is launch-bound, batching pays: 32 and 64 events on 8x8x4
+        # in one context against one event alone
+        print("# small-mesh gradient legs, plain and 32 / 64 events batched", file=sys.stderr, flush=True)
         sp = gradient_leg(k=args.checkpoint_every, workload="gradient_8x8x4")
-        sb = gradient_leg(k=args.checkpoint_every, n_events=32, workload="gradient_8x8x4")
-        sb["per_event_speedup_over_plain"] = sp["s_per_event_gradient"] / sb["s_per_event_gradient"]
         out["gradient_small_plain"] = sp
-        out["gradient_small_batched"] = sb
+        for ne, key in ((32, "gradient_small_batched"), (64, "gradient_small_batched64")):
+            sb = gradient_leg(k=args.checkpoint_every, n_events=ne, workload="gradient_8x8x4")
+            sb["per_event_speedup_over_plain"] = sp["s_per_event_gradient"] / sb["s_per_event_gradient"]
+            out[key] = sb
     if not args.no_batched:
         print("# batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg()

@@ -2165,6 +2165,30 @@ void *scratch_address(int device) {
     return device >= 0 && (size_t)device < g_scratch_ptr.size() ? g_scratch_ptr[(size_t)device] : nullptr;
 }
 
+// Outcome of the exhaustive reciprocal-quotient check (k_verify_div) per
+// device and divisor bit pattern: a pure function of the divisor, its
+// reciprocal RN(1/b) and this build, so a process runs it once per divisor
+// and device instead of at every init.
+std::mutex g_div_mu;
+std::map<std::pair<int, uint32_t>, unsigned long long> g_div_bad;
+
+bool div_cached(int device, float b, unsigned long long *bad) {
+    uint32_t u;
+    std::memcpy(&u, &b, 4);
+    std::lock_guard<std::mutex> lock(g_div_mu);
+    const auto it = g_div_bad.find({device, u});
+    if (it == g_div_bad.end()) return false;
+    *bad = it->second;
+    return true;
+}
+
+void div_store(int device, float b, unsigned long long bad) {
+    uint32_t u;
+    std::memcpy(&u, &b, 4);
+    std::lock_guard<std::mutex> lock(g_div_mu);
+    g_div_bad[{device, u}] = bad;
+}
+
 // A projection's block tables and its device buffer's extent: the field
 // (reference layout), the output, then per axis the block starts, element
 // indices and integrals.
@@ -2454,20 +2478,36 @@ int ses3d_hip_init(ses3d_hip_context *c) {
     M.LUT = c->LUT;
     M.PMLX = c->PMLX; M.PMLY = c->PMLY; M.PMLZ = c->PMLZ;
     {
-        // reciprocal-based quotients: verified exhaustively on this device
+        // reciprocal-based quotients: verified exhaustively on this device,
+        // once per divisor in the process
         float bh[4] = {c->cfg.tau_p[0], c->cfg.tau_p[1], c->cfg.tau_p[2], 3.0f};
         float yh[4];
         for (int k = 0; k < 4; ++k) yh[k] = 1.0f / bh[k];
-        if (!c->div_buf) TRY(dalloc(c, &c->div_buf, 8));
-        float *bd = (float *)c->div_buf, *yd = bd + 4;
-        unsigned long long *badd = c->div_buf + 4;
-        TRY(upload(c, bd, bh, sizeof(bh)));
-        TRY(upload(c, yd, yh, sizeof(yh)));
-        HIP_TRY(hipMemsetAsync(badd, 0, sizeof(unsigned long long) * 4, c->stream));
-        hipLaunchKernelGGL(k_verify_div, dim3(8192), dim3(256), 0, c->stream, bd, yd, 4, badd);
-        HIP_TRY(hipGetLastError());
         unsigned long long bad[4];
-        TRY(download(c, bad, badd, sizeof(bad)));
+        float bn[4], yn[4];
+        int nn = 0, which[4];
+        for (int k = 0; k < 4; ++k)
+            if (!div_cached(c->device, bh[k], &bad[k])) {
+                bn[nn] = bh[k];
+                yn[nn] = yh[k];
+                which[nn++] = k;
+            }
+        if (nn) {
+            if (!c->div_buf) TRY(dalloc(c, &c->div_buf, 8));
+            float *bd = (float *)c->div_buf, *yd = bd + 4;
+            unsigned long long *badd = c->div_buf + 4;
+            TRY(upload(c, bd, bn, sizeof(float) * nn));
+            TRY(upload(c, yd, yn, sizeof(float) * nn));
+            HIP_TRY(hipMemsetAsync(badd, 0, sizeof(unsigned long long) * 4, c->stream));
+            hipLaunchKernelGGL(k_verify_div, dim3(8192), dim3(256), 0, c->stream, bd, yd, nn, badd);
+            HIP_TRY(hipGetLastError());
+            unsigned long long got[4];
+            TRY(download(c, got, badd, sizeof(got)));
+            for (int j = 0; j < nn; ++j) {
+                bad[which[j]] = got[j];
+                div_store(c->device, bn[j], got[j]);
+            }
+        }
         for (int k = 0; k < 3; ++k) M.rtau_p[k] = yh[k];
         M.r3 = yh[3];
         M.fast_tau = c->cfg.is_diss && bad[0] == 0 && bad[1] == 0 && bad[2] == 0;
