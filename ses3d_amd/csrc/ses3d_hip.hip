@@ -13,9 +13,9 @@
 //                                            block (gx>>2, gy>>2, gz>>2) at (gz&3)*16+(gy&3)*4+(gx&3);
 //                                            (vx, vy) as one pair plane, vz alone
 // Per step (SOURCE/ses3d_evolution.f90:7-373):
-//   k_receivers  seismograms from the pre-update velocity (ses3d_output.f90:475-495),
-//                one workgroup per receiver
-//   k_element    gather v -> 9 velocity gradients (5-term GLL contractions through LDS)
+//   k_element    (recording steps) its first workgroups sample the seismograms from the
+//                pre-update velocity (ses3d_output.f90:475-495), one per receiver; the
+//                others: gather v -> 9 velocity gradients (5-term GLL contractions through LDS)
 //                -> spherical terms -> displacement-gradient update -> absorbing-layer terms
 //                -> stress rates (elastic or 3-mechanism anelastic) -> split-stress update
 //                -> weak-form divergence (LDS) + moment source -> point forces / adjoint
@@ -131,6 +131,20 @@ struct ForceDev {
     int comp_mask;        // bit c: subtract from component c
     const float *w;       // 125 weights
     const float *stf;     // 3*nt: component c at stf[c*nt + it-1]
+};
+
+// A receiver (SOURCE/ses3d_output.f90:456-497): the event's global velocity
+// it samples, its element, its 125 interpolation weights (index l + 5m + 25n,
+// precomputed on the host) and its 3 x nt seismogram (component-major).
+struct RecDev {
+    const float *v;     // (vx, vy) pair plane, vz at 2*ngb
+    const int *itp;     // the section's step number
+    long long ngb;
+    int nbx, nby;
+    int ex, ey, ez;
+    int nt;
+    const float *w;
+    float *out;
 };
 
 struct EventDev {
@@ -334,6 +348,32 @@ __device__ __forceinline__ long long gidx(int gx, int gy, int gz, int NBX, int N
            (((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
 }
 
+// Seismogram sample of one receiver (SOURCE/ses3d_output.f90:477-491) by one
+// workgroup of >= 128 lanes: lane t forms the product of term t of the
+// reference's sum (i slowest, k fastest) for the three components, then one
+// lane per component adds the 125 products in that order from 0, so every
+// product and every partial sum is rounded as in the sequential loop while the
+// 125 gathers are issued at once.  `prod`: 3 x 128 floats of LDS.
+__device__ __forceinline__ void receiver_sample(const RecDev &R, float *prod) {
+    const int t = threadIdx.x;
+    if (t < 125) {
+        const int i = t / 25, j = (t / 5) % 5, k = t % 5;
+        const long long g = gidx(4 * R.ex + i, 4 * R.ey + j, 4 * R.ez + k, R.nbx, R.nby);
+        const f2v vxy = *(cgf2v *)(R.v + 2 * g);
+        const float w = R.w[i + 5 * (j + 5 * k)];
+        prod[t] = vxy.x * w;
+        prod[128 + t] = vxy.y * w;
+        prod[256 + t] = R.v[2 * R.ngb + g] * w;
+    }
+    __syncthreads();
+    if (t < 3) {
+        const int it = *R.itp;
+        float acc = 0.f;
+        for (int q = 0; q < 125; ++q) acc = acc + prod[t * 128 + q];
+        R.out[t * R.nt + it - 1] = acc;
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of tiles so neighbouring elements share an L2.
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
@@ -394,7 +434,7 @@ __device__ __forceinline__ void subtract_forces(FT *__restrict__ F, const int co
 // MT / ET: MeshDev / EventDev, plain or in the constant address space.
 template <int DISS, int MODE, int ANISO, int EPB, int NT, typename MT, typename ET>
 __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const int it, const int blk,
-                                             const int blocks_per_event) {
+                                             const int blocks_per_event, const RecDev *rec) {
     const int ev = blk / blocks_per_event;
     const int eb = blk - ev * blocks_per_event;
     auto &E = evs[ev];
@@ -417,6 +457,11 @@ __device__ __forceinline__ void element_step(MT &Ms, ET *__restrict__ evs, const
     __shared__ float hv[MODE ? EPB : 1][MODE ? 9 * TPE : 1];
     __shared__ float sv[EPB][3][TPE];
     __shared__ float ss[EPB][9][TPE];
+    if (rec) {
+        // a receiver workgroup of a recording step (uniform), in the stress buffer's LDS
+        receiver_sample(*rec, &ss[0][0][0]);
+        return;
+    }
     for (int k = t; k < 7 * 25 + 40; k += EPB * TPE) tabs[k] = Ms.tab[k];
     const bool hcomp = MODE != 0 && Ms.hdeg < 4;   // uniform
     const float *D1 = tabs, *D2 = tabs + 25, *D3 = tabs + 50, *WDL = tabs + 75;
@@ -873,9 +918,13 @@ template <int DISS, int MODE, int ANISO, int EPB, int NT>
 __global__ __launch_bounds__(EPB * TPE)
 __attribute__((amdgpu_waves_per_eu(WAVES_MIN(MODE, EPB), 8)))
 void k_element(const MeshDev *__restrict__ mesh, const EventDev *__restrict__ evs, const int *itp,
-               int blocks_per_event, int remap) {
-    const int blk = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    element_step<DISS, MODE, ANISO, EPB, NT>(*mesh, evs, *itp, blk, blocks_per_event);
+               int blocks_per_event, int remap, const RecDev *__restrict__ recs, int nrec) {
+    // the first nrec workgroups sample the receivers (recording steps), the
+    // others advance the elements; both only read the velocity
+    const int b = (int)blockIdx.x - nrec;
+    const RecDev *rec = b < 0 ? recs + blockIdx.x : nullptr;
+    const int blk = b < 0 ? 0 : remap ? xcd_remap(b, (int)gridDim.x - nrec) : b;
+    element_step<DISS, MODE, ANISO, EPB, NT>(*mesh, evs, *itp, blk, blocks_per_event, rec);
 }
 
 // Several sections (meshes of any size, each with its own events and step
@@ -907,11 +956,14 @@ __device__ __forceinline__ int section_of(cSectionDev *secs, int nsec, int b, bo
 template <int DISS, int MODE, int ANISO, int EPB, int NT>
 __global__ __launch_bounds__(EPB * TPE)
 __attribute__((amdgpu_waves_per_eu(WAVES_MIN(MODE, EPB), 8)))
-void k_element_group(const SectionDev *__restrict__ secs, int nsec, int remap) {
-    const int b = remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+void k_element_group(const SectionDev *__restrict__ secs, int nsec, int remap, const RecDev *__restrict__ recs,
+                     int nrec) {
+    const int r = (int)blockIdx.x - nrec;
+    const int b = r < 0 ? 0 : remap ? xcd_remap(r, (int)gridDim.x - nrec) : r;
     cSectionDev *cs = (cSectionDev *)secs;
     cSectionDev &S = cs[section_of(cs, nsec, b, false)];
-    element_step<DISS, MODE, ANISO, EPB, NT>(*(cMeshDev *)S.mesh, (cEventDev *)S.evs, *S.itp, b - S.eb0, S.bpe);
+    element_step<DISS, MODE, ANISO, EPB, NT>(*(cMeshDev *)S.mesh, (cEventDev *)S.evs, *S.itp, b - S.eb0, S.bpe,
+                                             r < 0 ? recs + blockIdx.x : nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1154,46 +1206,6 @@ __global__ __launch_bounds__(DEFAULT_NODE_THREADS) void k_mass(MeshDev *mesh, fl
     MM[gt] = mm;
 }
 
-// seismograms (SOURCE/ses3d_output.f90:477-491): one thread per (event, receiver,
-// component); weights precomputed per receiver; summed in the reference's i,j,k order.
-struct RecDev {
-    int ev;
-    int ex, ey, ez;
-    const float *w;     // 125 weights, index l + 5m + 25n
-    float *out;         // 3 x nt for this receiver (component-major)
-};
-
-// One workgroup per receiver: lane t forms the product of term t of the
-// reference's sum (i slowest, k fastest) for the three components, then one
-// lane per component adds the 125 products in that order from 0, so every
-// product and every partial sum is rounded as in the sequential loop while
-// the 125 gathers are issued at once.
-__global__ __launch_bounds__(128) void k_receivers(const MeshDev *__restrict__ mesh,
-                                                   const EventDev *__restrict__ evs,
-                                                   const RecDev *__restrict__ recs, int nt, const int *itp) {
-    __shared__ float prod[3][128];
-    const int it = *itp;
-    const RecDev R = recs[blockIdx.x];
-    const MeshDev &Ms = *mesh;
-    const int t = threadIdx.x;
-    if (t < 125) {
-        const int i = t / 25, j = (t / 5) % 5, k = t % 5;
-        const long long g = gidx(4 * R.ex + i, 4 * R.ey + j, 4 * R.ez + k, Ms.NBX, Ms.NBY);
-        const float *v = evs[R.ev].v;
-        const f2v vxy = *(cgf2v *)(v + 2 * g);   // (vx, vy) pair plane, then vz
-        const float w = R.w[i + 5 * (j + 5 * k)];
-        prod[0][t] = vxy.x * w;
-        prod[1][t] = vxy.y * w;
-        prod[2][t] = v[2 * Ms.NGB + g] * w;
-    }
-    __syncthreads();
-    if (t < 3) {
-        float acc = 0.f;
-        for (int q = 0; q < 125; ++q) acc = acc + prod[t][q];
-        R.out[t * nt + it - 1] = acc;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -1240,8 +1252,8 @@ struct StepRunner {
 
     virtual ~StepRunner() {}
     // the kernels of one step for every event, reading the step number from HBM
-    virtual int launch_receivers(bool record) = 0;
-    virtual int launch_element(int mode) = 0;
+    // the element kernel of one step; `record` adds the receivers' workgroups
+    virtual int launch_element(int mode, bool record) = 0;
     virtual int launch_node() = 0;
     virtual int set_step(int it) = 0;   // store it in every step counter, stream-ordered
     virtual int schedule_changed() {
@@ -1320,6 +1332,7 @@ struct ses3d_hip_context : StepRunner {
     EventDev *ev_d = nullptr;
     // receivers / forces (all events)
     RecDev *recs_d = nullptr;
+    std::vector<RecDev> recs_h;   // the table on the device (groups concatenate them)
     int nrec_total = 0;
     ForceDev *forces_d = nullptr;
     size_t recs_cap = 0, forces_cap = 0;
@@ -1344,10 +1357,8 @@ struct ses3d_hip_context : StepRunner {
     std::vector<int> slot_dev;                     // the table currently on the device
     std::vector<EventDev> ev_dev;                  // the event table currently on the device
 
-    int launch_receivers_on(hipStream_t st, bool record);
     int set_step_on(hipStream_t st, int it);
-    int launch_receivers(bool record) override { return launch_receivers_on(stream, record); }
-    int launch_element(int mode) override;
+    int launch_element(int mode, bool record) override;
     int launch_node() override;
     int set_step(int it) override { return set_step_on(stream, it); }
 };
@@ -1359,6 +1370,9 @@ struct ses3d_hip_group : StepRunner {
     std::vector<int> gens;            // member generations the launch table was built for
     SectionDev *secs_d = nullptr;
     int elem_wgs = 0, node_wgs = 0;   // workgroups of one group launch
+    RecDev *recs_d = nullptr;         // every member's receivers
+    size_t recs_cap = 0;
+    int nrec = 0;
     int diss = 0, aniso = 0, af = 0;
 
     int build_table();
@@ -1366,14 +1380,7 @@ struct ses3d_hip_group : StepRunner {
         drop_graphs();
         return build_table();
     }
-    int launch_receivers(bool record) override {
-        for (auto *c : ctx) {
-            int rc = c->launch_receivers_on(stream, record);
-            if (rc) return rc;
-        }
-        return 0;
-    }
-    int launch_element(int mode) override;
+    int launch_element(int mode, bool record) override;
     int launch_node() override;
     int set_step(int it) override {
         for (auto *c : ctx) {
@@ -1529,16 +1536,18 @@ struct ElemLaunch {
     const SectionDev *secs;
     int nsec;
     bool nt;    // state streams non-temporal
+    const RecDev *recs;   // receivers sampled by the first nrec workgroups (recording steps)
+    int nrec;
 };
 
 template <int D, int MO, int AN, int EP, int NT>
 void launch_elem_nt(const ElemLaunch &L) {
     if (L.secs)
-        hipLaunchKernelGGL((k_element_group<D, MO, AN, EP, NT>), dim3(L.wgs), dim3(EP * TPE), 0, L.stream, L.secs,
-                           L.nsec, L.remap);
+        hipLaunchKernelGGL((k_element_group<D, MO, AN, EP, NT>), dim3(L.wgs + L.nrec), dim3(EP * TPE), 0, L.stream,
+                           L.secs, L.nsec, L.remap, L.recs, L.nrec);
     else
-        hipLaunchKernelGGL((k_element<D, MO, AN, EP, NT>), dim3(L.wgs), dim3(EP * TPE), 0, L.stream, L.mesh, L.evs,
-                           L.itp, L.bpe, L.remap);
+        hipLaunchKernelGGL((k_element<D, MO, AN, EP, NT>), dim3(L.wgs + L.nrec), dim3(EP * TPE), 0, L.stream, L.mesh,
+                           L.evs, L.itp, L.bpe, L.remap, L.recs, L.nrec);
 }
 
 template <int D, int MO, int AN, int EP>
@@ -1600,7 +1609,6 @@ Schedule trial_schedule(int k) {
 // element and node kernels on the runner's stream; they are read back after
 // the run, so the stream is never stalled.
 int launch_step(StepRunner *r, int mode, bool record, bool timed) {
-    TRY(r->launch_receivers(record));
     hipEvent_t *ev = nullptr;
     if (timed) {
         const size_t need = 4 * (size_t)(r->n_elem + 1);
@@ -1612,7 +1620,7 @@ int launch_step(StepRunner *r, int mode, bool record, bool timed) {
         ev = &r->tev[4 * (size_t)r->n_elem];
         HIP_TRY(hipEventRecord(ev[0], r->stream));
     }
-    TRY(r->launch_element(mode));
+    TRY(r->launch_element(mode, record));
     if (ev) HIP_TRY(hipEventRecord(ev[1], r->stream));
     if (ev) HIP_TRY(hipEventRecord(ev[2], r->stream));
     TRY(r->launch_node());
@@ -1729,24 +1737,16 @@ int autotune_steps(StepRunner *r, int it_begin, int n, int mode, bool record) {
 
 }  // namespace
 
-int ses3d_hip_context::launch_receivers_on(hipStream_t st, bool record) {
-    if (!(record && nrec_total > 0)) return 0;
-    hipLaunchKernelGGL(k_receivers, dim3(nrec_total), dim3(128), 0, st, mesh_d, ev_d, recs_d, cfg.nt, it_d);
-    HIP_TRY(hipGetLastError());
-    return 0;
-}
-
-
 int ses3d_hip_context::set_step_on(hipStream_t st, int it) {
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, st, it_d, it);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int ses3d_hip_context::launch_element(int mode) {
+int ses3d_hip_context::launch_element(int mode, bool record) {
     const int bpe = elem_blocks(this, sched.epb);
     ElemLaunch L{stream, sched.epb, sched.remap_elem, (unsigned)(bpe * n_events), mesh_d, ev_d, it_d, bpe, nullptr, 0,
-                 use_streaming(this)};
+                 use_streaming(this), recs_d, record ? nrec_total : 0};
     return launch_elem(cfg.is_diss, mode, aniso ? 1 : 0, L);
 }
 
@@ -1778,6 +1778,21 @@ int ses3d_hip_group::build_table() {
     }
     elem_wgs = (int)eb;
     node_wgs = (int)nb;
+    // every member's receivers, sampled by the first workgroups of the element launch
+    std::vector<RecDev> r;
+    for (auto *c : ctx) r.insert(r.end(), c->recs_h.begin(), c->recs_h.end());
+    if (eb + (long long)r.size() >= (1LL << 31)) return fail(SES3D_HIP_EINVAL, "group too large for one launch");
+    if (r.size() > recs_cap) {
+        if (recs_d) HIP_TRY(hipFree(recs_d));
+        recs_d = nullptr;
+        recs_cap = 0;
+        if (hipMalloc((void **)&recs_d, sizeof(RecDev) * r.size()) != hipSuccess)
+            return fail(SES3D_HIP_ENOMEM, "hipMalloc of the group receiver table failed");
+        recs_cap = r.size();
+    }
+    nrec = (int)r.size();
+    if (!r.empty())
+        HIP_TRY(hipMemcpyAsync(recs_d, r.data(), sizeof(RecDev) * r.size(), hipMemcpyHostToDevice, stream));
     gens.clear();
     for (auto *c : ctx) gens.push_back(c->generation);
     HIP_TRY(hipMemcpyAsync(secs_d, h.data(), sizeof(SectionDev) * h.size(), hipMemcpyHostToDevice, stream));
@@ -1785,9 +1800,9 @@ int ses3d_hip_group::build_table() {
     return 0;
 }
 
-int ses3d_hip_group::launch_element(int mode) {
+int ses3d_hip_group::launch_element(int mode, bool record) {
     ElemLaunch L{stream, sched.epb, sched.remap_elem, (unsigned)elem_wgs, nullptr, nullptr, nullptr, 0, secs_d,
-                 (int)ctx.size(), use_streaming(this)};
+                 (int)ctx.size(), use_streaming(this), recs_d, record ? nrec : 0};
     return launch_elem(diss, mode, aniso, L);
 }
 
@@ -2556,7 +2571,9 @@ int ses3d_hip_init(ses3d_hip_context *c) {
             for (int r = 0; r < nr; ++r) {
                 ses3d::point_weights(E.recs[r].xl, E.recs[r].yl, E.recs[r].zl, &w[(size_t)r * 125]);
                 RecDev R;
-                R.ev = ev; R.ex = E.recs[r].ex; R.ey = E.recs[r].ey; R.ez = E.recs[r].ez;
+                R.v = E.v; R.itp = c->it_d; R.ngb = c->NGB; R.nbx = c->NBX; R.nby = c->NBY;
+                R.ex = E.recs[r].ex; R.ey = E.recs[r].ey; R.ez = E.recs[r].ez;
+                R.nt = nt;
                 R.w = E.rec_w + (size_t)r * 125;
                 R.out = E.seis + (size_t)r * 3 * nt;
                 recs.push_back(R);
@@ -2625,6 +2642,7 @@ int ses3d_hip_init(ses3d_hip_context *c) {
         }
     }
     c->nrec_total = (int)recs.size();
+    c->recs_h = recs;
     if (!recs.empty()) {
         TRY(dfit(c, &c->recs_d, c->recs_cap, recs.size()));
         TRY(upload(c, c->recs_d, recs.data(), sizeof(RecDev) * recs.size()));
@@ -3211,6 +3229,7 @@ void ses3d_hip_group_destroy(ses3d_hip_group *g) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     g->release_runner();
     if (g->secs_d) (void)hipFree(g->secs_d);
+    if (g->recs_d) (void)hipFree(g->recs_d);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }
