@@ -109,6 +109,46 @@ def test_group_forward_bitexact(tmp_path):
         s.close()
 
 
+def test_group_members_with_and_without_receivers(tmp_path):
+    """Receivers are sampled by workgroups ahead of the elements in the
+    element launch; a group concatenates its members' receivers.  A member
+    whose events record nothing (and one recording event beside a silent one)
+    must leave every field and seismogram as in single-context runs."""
+    from ses3d_amd.solver import Group
+    nt = 30
+    runs = _sections(tmp_path, nt)
+
+    def build():
+        sols = []
+        for i, r in enumerate(runs):
+            s = _gpu(r)
+            if i == 1:
+                s.set_receivers(0, np.zeros((0, 3), np.float32))    # member without receivers
+            if i == 0:
+                s.set_receivers(1, np.zeros((0, 3), np.float32))    # second event of member 0 silent
+            s.init()
+            sols.append(s)
+        return sols
+
+    alone = build()
+    for s in alone:
+        s.run()
+    grouped = build()
+    print("group of 3 members, receivers on 2 of 4 events", flush=True)
+    with Group(grouped) as g:
+        g.run(1, 20)
+        g.run(21, nt - 20)
+    assert alone[1].nr(0) == 0 and alone[0].nr(1) == 0 and alone[0].nr(0) > 0
+    for a, b in zip(alone, grouped):
+        for k in range(a.n_events):
+            if a.nr(k):
+                _assert_same(b.seismograms(k), a.seismograms(k), f"seismograms ev{k}")
+            for c in ("vx", "vy", "vz"):
+                _assert_same(b.field(c + "_global", k), a.field(c + "_global", k), f"{c} ev{k}")
+    for s in alone + grouped:
+        s.close()
+
+
 def test_group_rejects_mixed_members(tmp_path):
     from ses3d_amd.solver import Group, Ses3dError
     a = _gpu(make_case(str(tmp_path / "a"), nel=(3, 3, 3), nt=8, is_diss=True, seed=1))
