@@ -397,6 +397,48 @@ def batched_leg(n_events: int = 32, steps: int = 30) -> dict:
             "schedule": {k: v for k, v in sched.items() if k != "trial_ms"}, "seismograms_finite": bool(ok)}
 
 
+def size_sweep(sizes=((16, 16, 8), (24, 24, 12), (32, 32, 16), (48, 48, 24), (64, 64, 32), (80, 80, 40),
+                      (96, 96, 48), (112, 112, 56), (128, 128, 64))) -> list:
+    """Plain forward step against section size, anelastic (3D random model,
+    pml=3) and elastic (1D, pml=2), one event with 20 receivers: 18 autotuning
+    steps, 30 steps timed kernel by kernel with HIP events, 16 steps that
+    capture the step graph, then 80 graph-replayed steps timed as a whole."""
+    from ses3d_amd import configs
+    from ses3d_amd.solver import Solver
+    out = []
+    for diss in (True, False):
+        for nel in sizes:
+            w = configs.Workload(f"sweep_{nel[0]}x{nel[1]}x{nel[2]}", tuple(nel), diss, 3 if diss else 2, 0.1, 150,
+                                 20, "grf" if diss else "none")
+            run = configs.build(w, seed=0)
+            s = Solver.from_run(run, adjoint_flag=0, pml=w.pml)
+            s.set_model(run.model)
+            s.load_event(0, run.events[0])
+            s.init()
+            del run
+            sch = s.autotune(1, 18)
+            s.set_profiling(True)
+            s.run(19, 30)
+            tm = s.timing()
+            s.set_profiling(False)
+            s.run(49, 16)
+            wall = _timed_run(s, 65, 80)
+            ok = bool(np.isfinite(s.seismograms()).all())
+            s.close()
+            ab = algorithmic_bytes(nel, diss)
+            n_p = w.n_points
+            e = tm["element_ms"] / tm["n_element"]
+            r = {"elements": list(nel), "anelastic": diss, "gll_points": n_p, "ms_per_step": 1e3 * wall / 80,
+                 "updates_per_s": n_p * 80 / wall, "element_ms": e, "node_ms": tm["node_ms"] / tm["n_element"],
+                 "element_frac": ab["element"] * n_p / (e * 1e-3) / HBM_PEAK,
+                 "step_frac": ab["step"] * n_p * 80 / wall / HBM_PEAK,
+                 "schedule": {k: sch[k] for k in ("epb", "node_threads", "remap_elem", "remap_node")},
+                 "seismograms_finite": ok}
+            print(json.dumps(r), flush=True)
+            out.append(r)
+    return out
+
+
 def inversion_leg(n_events: int = 16, nt: int = 4000, k: int = 100, step_frac: float = 0.005) -> dict:
     """One inversion iteration on the 64x64x32 anelastic mesh: data from the
     'true' model (1D + 2% Gaussian blobs of 300 km) for n_events events, the
@@ -501,6 +543,8 @@ def main():
                     help="only the small-mesh gradient legs: one event alone and 32 events in one context, "
                          "on 8x8x4 and 16x16x8 elements")
     ap.add_argument("--nt", type=int, default=None, help="time steps of the --small-gradients legs")
+    ap.add_argument("--size-sweep", action="store_true",
+                    help="plain forward step time and roofline fraction against section size, then exit")
     ap.add_argument("--graph-steps", type=int, default=None,
                     help="steps per captured step graph in every context (0: direct launches)")
     args = ap.parse_args()
@@ -528,6 +572,9 @@ def main():
         print(json.dumps({"copy_bytes_each_way": 4 * n, "launches": 5}))
         return
 
+    if args.size_sweep:
+        size_sweep()
+        return
     if args.inversion:
         print(json.dumps(inversion_leg()))
         return
