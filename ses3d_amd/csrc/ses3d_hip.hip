@@ -1781,8 +1781,10 @@ int ses3d_hip_group::build_table() {
     // every member's receivers, sampled by the first workgroups of the element launch
     std::vector<RecDev> r;
     for (auto *c : ctx) r.insert(r.end(), c->recs_h.begin(), c->recs_h.end());
-    if (eb + (long long)r.size() >= (1LL << 31)) return fail(SES3D_HIP_EINVAL, "group too large for one launch");
+    if ((eb + (long long)r.size()) * sched.epb * TPE >= (1LL << 32) || nb * sched.node_threads >= (1LL << 32))
+        return fail(SES3D_HIP_EINVAL, "group too large for one launch");
     if (r.size() > recs_cap) {
+        HIP_TRY(hipStreamSynchronize(stream));   // no launch of the group still reads the old table
         if (recs_d) HIP_TRY(hipFree(recs_d));
         recs_d = nullptr;
         recs_cap = 0;
@@ -2641,6 +2643,10 @@ int ses3d_hip_init(ses3d_hip_context *c) {
             for (int it = 1; it <= nt; ++it) c->saving_vector[it] = E.slot_h[it] >= 0 ? it : 0;
         }
     }
+    // one element launch: the elements' workgroups of every event and one
+    // workgroup per receiver, 128 to 512 lanes each, below 2^32 lanes
+    if (((long long)c->NE * c->n_events + 4LL * ((long long)recs.size() + c->n_events)) * TPE >= (1LL << 32))
+        return fail(SES3D_HIP_EINVAL, "events x elements plus receivers exceed one launch: split the events over contexts");
     c->nrec_total = (int)recs.size();
     c->recs_h = recs;
     if (!recs.empty()) {
