@@ -397,6 +397,14 @@ def make_boxes(setup: Setup) -> List[Box]:
 
 def write_boxfile(path: str, setup: Setup) -> List[Box]:
     boxes = make_boxes(setup)
+    write_boxes(path, boxes, setup.px, setup.py, setup.pz)
+    return boxes
+
+
+def write_boxes(path: str, boxes: Sequence[Box], px: int, py: int, pz: int) -> None:
+    """MODELS/MODELS/boxfile (MODELS/SOURCE/generate_models.f90:208-282):
+    14 header lines, the process counts, then per box its index, multi-index,
+    index bounds and physical bounds (values written so they read back exactly)."""
     hdr = ["- format description -", "total number of processes",
            "number of processes in x direction", "number of processes in y direction",
            "number of processes in z direction", "single index (npz-1)*py*px+(npy-1)*px+npx",
@@ -405,8 +413,7 @@ def write_boxfile(path: str, setup: Setup) -> List[Box]:
            "physical boundaries x", "physical boundaries y", "phyiscal boundaries z",
            "-------------------"]
     out = [" " + h for h in hdr]
-    out += [f" {setup.px * setup.py * setup.pz}", f" {setup.px}", f" {setup.py}", f" {setup.pz}",
-            " -------------------"]
+    out += [f" {px * py * pz}", f" {px}", f" {py}", f" {pz}", " -------------------"]
     for b in boxes:
         out += [f" {b.index}", f" {b.multi[0]} {b.multi[1]} {b.multi[2]}",
                 f" {b.nx_min} {b.nx_max}", f" {b.ny_min} {b.ny_max}", f" {b.nz_min} {b.nz_max}",
@@ -414,7 +421,6 @@ def write_boxfile(path: str, setup: Setup) -> List[Box]:
                 " -------------------"]
     with open(path, "w") as fh:
         fh.write("\n".join(out) + "\n")
-    return boxes
 
 
 def read_boxfile(path: str) -> List[Box]:

@@ -9,8 +9,9 @@ The roles it plays follow MODELS/SOURCE/generate_models.f90 (boxfile and model
 files, :200-452), MODELS/MODELS_1D/prem_iso.f90 (a 1D isotropic earth) and
 MODELS/SOURCE/add_perturbation.f90 (3D perturbations of vp/vs/rho).  The 1D
 profile here is a simplified piecewise-linear PREM-like model, not PREM's
-polynomial coefficients; the generator only needs to produce inputs that both
-the reference and this solver consume identically.
+polynomial coefficients (the benchmark workloads' 1D model); the reference's
+own generator, its 1D models and its block perturbations are reproduced bit
+for bit in ses3d_amd/generator.py.
 """
 from __future__ import annotations
 
