@@ -68,13 +68,20 @@ class Problem:
     ``run``: a RunInput (geometry, relaxation, events); ``data``: one observed
     seismogram array (3, nr, nt) per event, receivers in the order the solver
     locates them.  Events are solved ``events_per_batch`` at a time in one
-    context (they advance in the same launches)."""
+    context (they advance in the same launches).
+
+    ``measure``: optional callable ``(ev, u, dt) -> (chi_ev, rows)`` that
+    replaces the L2 waveform misfit: ``u`` is the event's synthetic velocity
+    seismograms (3, nr, nt), ``rows`` the adjoint-source rows (nr, nt, 3),
+    time-reversed as ADJOINT/<ev>/ad_src_<k> stores them (for instance
+    ses3d_amd.adjoint_source.event_rows of the reference's measurements)."""
 
     def __init__(self, run, data: Sequence[np.ndarray], pml: int = 2, ckpt: int = 100,
                  events_per_batch: Optional[int] = None, q_model: Optional[np.ndarray] = None,
                  taper_width: float = 0.0, progress=None, history_degree: int = 4,
-                 kernel_bits: int = 64, ckpt_keep: int = 1):
+                 kernel_bits: int = 64, ckpt_keep: int = 1, measure=None):
         self.run = run
+        self.measure = measure
         self.data = [np.asarray(d, np.float32) for d in data]
         self.pml = pml
         self.ckpt = ckpt
@@ -117,6 +124,14 @@ class Problem:
         if self.is_diss:
             out["Q"] = np.asarray(Q if Q is not None else self.q_model, np.float64)
         return {k: np.asfortranarray(v.astype(np.float32)) for k, v in out.items()}
+
+    def _measure(self, ev: int, u: np.ndarray, rows: bool):
+        """(chi of event ev, its adjoint rows or None)."""
+        if self.measure is not None:
+            chi, r = self.measure(ev, u, self.dt)
+            return float(chi), (np.ascontiguousarray(r, np.float32) if rows else None)
+        chi = MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
+        return chi, (MF.l2_adjoint_rows(u, self.data[ev], self.dt, taper=self.taper) if rows else None)
 
     def _batches(self):
         for b0 in range(0, self.n_events, self.events_per_batch):
@@ -167,8 +182,8 @@ class Problem:
                     u = s.seismograms(k)
                     self.last_seismograms.append(u)
                     st.seismograms[ev] = u
-                    chi += MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
-                    rows = MF.l2_adjoint_rows(u, self.data[ev], self.dt, taper=self.taper)
+                    c, rows = self._measure(ev, u, True)
+                    chi += c
                     st.sources[ev] = (self.run.events[ev].rec32[s.receiver_index(k)], rows)
                 if self.progress:
                     self.progress(f"forward solve of events {evs[0]}..{evs[-1]} done")
@@ -239,7 +254,7 @@ class Problem:
                 for k, ev in enumerate(evs):
                     u = s.seismograms(k)
                     self.last_seismograms.append(u)
-                    chi += MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
+                    chi += self._measure(ev, u, False)[0]
                 if self.progress:
                     self.progress(f"forward solve of events {evs[0]}..{evs[-1]} done")
         finally:
@@ -269,8 +284,8 @@ class Problem:
                 for k, ev in enumerate(evs):
                     u = f.seismograms(k)
                     self.last_seismograms.append(u)
-                    chi += MF.l2_misfit(u, self.data[ev], self.dt, self.taper)
-                    rows = MF.l2_adjoint_rows(u, self.data[ev], self.dt, taper=self.taper)
+                    c, rows = self._measure(ev, u, True)
+                    chi += c
                     a.set_adjoint_sources(k, self.run.events[ev].rec32[f.receiver_index(k)], rows)
                 a.init()
                 a.adjoint_checkpointed(f)
