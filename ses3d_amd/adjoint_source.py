@@ -361,3 +361,8 @@ def event_rows(u: np.ndarray, dt: float, make, rec_deg=None) -> np.ndarray:
         make(a, r)
         out[r] = a.rows()
     return out
+
+
+# the reference tools' class names (TOOLS/seismograms.py:11, TOOLS/adjoint_source.py:11)
+ses3d_seismogram = Seismogram
+adjoint_source = AdjointSource
