@@ -265,6 +265,19 @@ def build_ref() -> bool:
     return r.returncode == 0 and ref_available()
 
 
+def build_ref_multirank(per_rank) -> str:
+    """Build the reference solver for multi-rank runs whose rank boxes hold
+    per_rank = (ex, ey, ez) elements each (its static extents nx_max, ny_max,
+    nz_max set to ex-1, ey-1, ez-1, as the reference's workflow sets them per
+    run; `make ref-mr`).  Returns the binary's path."""
+    tag = "_".join(str(int(v) - 1) for v in per_rank)
+    exe = os.path.join(HERE, "_ref", f"ses3d_mr_{tag}.exe")
+    r = subprocess.run(["make", "-s", "-C", HERE, "ref-mr", f"MR={tag}"], capture_output=True, text=True)
+    if r.returncode != 0 or not os.path.exists(exe):
+        raise RuntimeError(f"reference build for {per_rank} elements per rank failed:\n{r.stdout[-2000:]}{r.stderr[-2000:]}")
+    return exe
+
+
 def run_ref(root: str, timeout: float = 600.0, nprocs: int = 1, exe: str = REF_EXE) -> subprocess.CompletedProcess:
     """Run the reference binary (or the grad_Q build, exe=REF_GRADQ_EXE) on a
     run tree (cwd = root/MAIN)."""

@@ -3,20 +3,29 @@ solver run on as many MPI ranks (oracle/_ref/ses3d.exe under the image's
 MPICH mpiexec).
 
 Case fixture_case.MULTIBOX: the section of `multibox_adjoint` (6x5x4
-elements, laterally varying elastic model, moment tensor) split 2 x 1 x 2.
+elements, laterally varying elastic model, moment tensor) split 2 x 1 x 2,
+run by the reference built for rank boxes of 3 x 5 x 2 elements
+(oracle/pyoracle.build_ref_multirank: its static extents nx_max, ny_max,
+nz_max = 2, 4, 1, as the reference's workflow sets them per run).  The
+extents must match: SOURCE/ses3d_miscellaneous.f90:156-166 declares the
+mass-matrix exchange's argument with the rank's own extents
+((nx+1)*lpd+1, ...) while the array passed is sized by the static ones
+(SOURCE/ses3d_modules.f90:105), so with larger static extents the exchange
+adds the neighbour's face masses at the wrong addresses and every rank-face
+node keeps half its mass (face velocities double: measured here).
 The fixture holds the single-rank outputs make_golden.py stores for any case
 (the drop-in driver run on the split tree must reproduce them bit for bit: the
 boxes are one section), and, from the 4-rank reference run of the split tree,
   mr_seismograms (3, nr, nt)      the receivers' files
   mr_shape_<r>                    the extent of rank r's field files
+  mr_<vx|vy|vz>_<r>               rank r's velocity snapshot at the last step
   mr_rel_seismograms              relative L2 distance of mr_seismograms from
                                   the reference's own single-rank run
-The reference's multi-rank runs are not its single-rank runs: besides the
+The reference's multi-rank run differs from its single-rank run by the
 rounding of per-rank geometry and of the cross-rank face sums
-(SOURCE/ses3d_miscellaneous.f90:257-368), ranks > 0 of an anelastic run use
-sum(D_p) = 0 (formed on rank 0 only, SOURCE/ses3d_input.f90:353), and waves
-crossing rank faces differ by up to O(1) in some splits; mr_rel_seismograms
-records the distance for this case.
+(SOURCE/ses3d_miscellaneous.f90:257-368); mr_rel_seismograms records that
+distance.  (Ranks > 0 of an anelastic run also use sum(D_p) = 0, formed on
+rank 0 only at SOURCE/ses3d_input.f90:353; this case is elastic.)
 
     python tests/golden/make_golden_multibox.py
 """
@@ -53,7 +62,12 @@ def make(name: str) -> str:
     try:
         FC.write_tree(out, root, decomp=decomp)
         boxes = F.read_boxfile(os.path.join(root, "MODELS", "MODELS", "boxfile"))
-        r = PO.run_ref(root, nprocs=nproc)
+        nel = FC.CASES[name][0]
+        if any(n % d for n, d in zip(nel, decomp)):
+            raise ValueError(f"{nel} elements do not split evenly into {decomp} rank boxes")
+        per_rank = tuple(n // d for n, d in zip(nel, decomp))
+        exe = PO.build_ref_multirank(per_rank)
+        r = PO.run_ref(root, nprocs=nproc, exe=exe)
         if r.returncode != 0:
             raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
         run = I.load_run(root)
@@ -67,14 +81,16 @@ def make(name: str) -> str:
         def rshape(b):
             return (b.nx + 1, b.ny + 1, b.nz + 1, 5, 5, 5)
 
-        def full(b):   # the reference's static extents (SOURCE/ses3d_modules.f90:13-15)
-            return (23, 28, 8, 5, 5, 5)
+        def full(b):   # the build's static extents (SOURCE/ses3d_modules.f90:13-15) = the rank's
+            return tuple(per_rank) + (5, 5, 5)
 
         for b in boxes:
             rk = b.index - 1
             out[f"mr_shape_{rk}"] = np.array(rshape(b))
             # the rank's snapshot file holds the reference's static extents
             assert os.path.getsize(os.path.join(ofd, f"vx_{rk}_{nt}")) == 8 + 4 * int(np.prod(full(b)))
+            for c in ("vx", "vy", "vz"):
+                out[f"mr_{c}_{rk}"] = F.read_field(os.path.join(ofd, f"{c}_{rk}_{nt}"), rshape(b))
         out["mr_decomp"] = np.array(decomp)
         print(f"{name}: {nproc} ranks, seismograms rel L2 to the single-rank run {out['mr_rel_seismograms']:.2e}")
         np.savez_compressed(path, **out)
