@@ -47,3 +47,17 @@ def test_boxes_that_do_not_tile_are_refused(tmp_path):
     boxes = F.read_boxfile(os.path.join(str(tmp_path), "MODELS", "MODELS", "boxfile"))
     with pytest.raises(ValueError):
         I.section_box(boxes[:-1])
+
+
+def test_multirank_fixture_agrees_with_the_single_rank_run():
+    """The fixture's 4-rank reference run (a build sized for its 3x5x2-element
+    rank boxes) differs from the reference's single-rank run only by the
+    rounding of per-rank geometry and face sums; a build with larger static
+    extents would leave every rank-face node with half its mass (face
+    velocities doubled, a distance of order 1e-3 here)."""
+    fx = dict(np.load(os.path.join(GOLDEN, "multibox_adjoint.npz")))
+    assert tuple(int(v) for v in fx["mr_decomp"]) == (2, 1, 2)
+    assert float(fx["mr_rel_seismograms"]) < 1e-7
+    rel = np.linalg.norm((fx["mr_seismograms"] - fx["seismograms"]).astype(np.float64)) / \
+        np.linalg.norm(fx["seismograms"].astype(np.float64))
+    assert rel < 1e-7
