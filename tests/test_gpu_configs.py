@@ -2,8 +2,9 @@
 oracle or HBM would not finish or fit), through size-independent properties.
 
 * config 1 (elastic 24x24x12, nt=2000): all 2000 steps bit-identical to the
-  float32 oracle (digests in tests/golden/elastic_24x24x12_oracle.npz; the
-  oracle reproduces the reference's float32 build bit for bit).
+  reference solver itself, run on one rank of a build sized for the section,
+  and to the float32 oracle, which agrees with it bit for bit
+  (tests/golden/elastic_24x24x12_oracle.npz, make_golden_config1.py).
 * config 3 (gradient on 48x48x24, anelastic): kernels bit-identical for
   checkpoint intervals k = 50, 100, 200 and with every segment's sampled
   steps kept (nothing replayed); the gradient against central finite
@@ -35,7 +36,7 @@ def _solver(run, af, w, nt=None, n=None):
     return s
 
 
-def test_config1_elastic_24x24x12_all_steps_match_oracle():
+def test_config1_elastic_24x24x12_all_steps_match_reference():
     from tests.golden.fixture_case import field_digest, input_digest
     fx = np.load(os.path.join(GOLDEN, "elastic_24x24x12_oracle.npz"))
     w = configs.WORKLOADS["elastic_24x24x12"]
@@ -51,6 +52,7 @@ def test_config1_elastic_24x24x12_all_steps_match_oracle():
     s.run(nt, 1)
     seis = s.seismograms()
     s.close()
+    np.testing.assert_array_equal(seis, fx["ref_seismograms"])
     assert hashlib.sha256(np.ascontiguousarray(seis, np.float32).tobytes()).hexdigest() == str(fx["seismograms_digest"])
     for c in ("vx", "vy", "vz"):
         assert field_digest(snap[c]) == str(fx[c + "_digest"]), c

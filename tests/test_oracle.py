@@ -163,6 +163,26 @@ def test_oracle_matches_reference_elastic_23x28x8(oracle_lib):
     np.testing.assert_array_equal(o.get("seismograms")[:, :, :200].astype(np.float32), fx["seismograms"][:, :, :200])
 
 
+def test_oracle_matches_reference_elastic_24x24x12(oracle_lib):
+    """Config 1's workload: the first 100 samples of the reference's 2000-step
+    seismograms (one rank of a build sized for 24x24x12 elements); the
+    fixture's oracle digest of all 2000 steps is the digest of the
+    reference's seismograms."""
+    import hashlib
+    from ses3d_amd import configs
+    from tests.cases import oracle_for
+    from tests.golden.fixture_case import input_digest
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "elastic_24x24x12_oracle.npz"))
+    assert hashlib.sha256(np.ascontiguousarray(fx["ref_seismograms"], np.float32).tobytes()).hexdigest() == \
+        str(fx["seismograms_digest"])
+    w = configs.WORKLOADS["elastic_24x24x12"]
+    run = configs.build(w, seed=0)
+    assert input_digest(run) == str(fx["digest"])
+    o = oracle_for(run, run.events[0], pml=w.pml)
+    o.run(1, 100)
+    np.testing.assert_array_equal(o.get("seismograms")[:, :, :100].astype(np.float32), fx["ref_seismograms"][:, :, :100])
+
+
 def test_rank_boxes_tile_the_section():
     """Rank boxes of a decomposed setup cover the section's elements exactly
     once, with the reference's bottom-up radial box order."""
