@@ -1,0 +1,103 @@
+"""Reference kernels of the gradient workload's mesh: 48x48x24 elements,
+anelastic, the 3D 'true' model (1D + 2% Gaussian blobs), the workload's
+moment source at 10 km depth and 100 surface receivers, dt=0.15 s, truncated
+to nt=300 (samp_ad=15), from the reference solver itself run on one rank of a
+build sized for the section (oracle/Makefile ref-mr MR=47_47_23: the forward
+pass with adjoint_flag=1, then the adjoint pass of the grad_Q build, which
+writes the four elastic kernels and grad_Q_mu).  The adjoint sources are the
+forward seismograms times 1e6 at the receivers, written as ADJOINT files.
+
+Stored: the forward seismograms (3, 100, 300) from the reference's files, the
+sha256 of each kernel (float32, Fortran layout), its L2 norm, and every
+KSTRIDE-th value of each kernel in Fortran order (for the float64-accumulator
+bound), with the digest of the inputs (rebuilt by ses3d_amd.configs.build
+with seed 0) to detect drift.
+
+    python tests/golden/make_golden_gradient_ref.py
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import sys
+import tempfile
+import time
+from dataclasses import replace
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import pyoracle as PO  # noqa: E402
+from ses3d_amd import configs, formats as F, inputs as I, models as M  # noqa: E402
+from tests.golden.fixture_case import field_digest, input_digest  # noqa: E402
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gradient_48x48x24_ref.npz")
+WORKLOAD = "gradient_48x48x24"
+NT = 300
+KSTRIDE = 421
+KERNELS = ("grad_rho", "grad_cp", "grad_csh", "grad_csv", "grad_Q_mu")
+
+
+def build_run():
+    return configs.build(configs.WORKLOADS[WORKLOAD], seed=0, nt=NT)
+
+
+def write_adjoint_files(root: str, run, seis: np.ndarray) -> None:
+    """ADJOINT/1: one adjoint source per receiver, the forward seismogram
+    times 1e6 (forward time), and the setup switched to adjoint_flag=2."""
+    e = run.events[0]
+    r = e.receivers
+    locs = np.stack([np.asarray(r.colat, np.float64), np.asarray(r.lon, np.float64),
+                     np.asarray(r.depth, np.float64)], 1)
+    d = os.path.join(root, "ADJOINT", "1")
+    os.makedirs(d, exist_ok=True)
+    F.write_ad_srcfile(os.path.join(d, "ad_srcfile"), locs)
+    for k in range(len(locs)):
+        F.write_ad_src(os.path.join(d, f"ad_src_{k + 1}"), locs[k], seis[:, k, :].astype(np.float64) * 1e6)
+    s = F.Setup.read(os.path.join(root, "INPUT", "setup"))
+    s.adjoint_flag = 2
+    s.write(os.path.join(root, "INPUT", "setup"))
+
+
+def main() -> str:
+    run = build_run()
+    e = run.events[0]
+    per_rank = (run.nx + 1, run.ny + 1, run.nz + 1)
+    exe = PO.build_ref_multirank(per_rank)
+    exe_q = PO.build_ref_multirank(per_rank, grad_q=True)
+    setup = replace(run.setup, adjoint_flag=1)
+    case = M.RunCase(setup, [e.event], [e.receivers], e.stf, run.model)
+    root = tempfile.mkdtemp(prefix="golden_gradient_ref_")
+    try:
+        M.write_run_tree(root, case)
+        t0 = time.time()
+        r = PO.run_ref(root, timeout=7200, exe=exe)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        print(f"reference forward: {NT} steps in {time.time() - t0:.0f} s", flush=True)
+        ofd = os.path.join(root, "MAIN", e.event.ofd)
+        seis = np.stack([F.read_seismogram(ofd, n)["trace"].astype(np.float32) for n in e.receivers.names], 1)
+        write_adjoint_files(root, run, seis)
+        t0 = time.time()
+        r = PO.run_ref(root, timeout=7200, exe=exe_q)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        print(f"reference adjoint (grad_Q build): {NT} steps in {time.time() - t0:.0f} s", flush=True)
+        shape = (run.nx + 1, run.ny + 1, run.nz + 1, 5, 5, 5)
+        out = {"nt": NT, "digest": input_digest(run), "seismograms": seis, "kstride": KSTRIDE}
+        for g in KERNELS:
+            k = F.read_field(os.path.join(ofd, f"{g}_0"), shape)
+            out[f"{g}_digest"] = field_digest(k)
+            out[f"{g}_norm"] = np.linalg.norm(k.astype(np.float64))
+            out[f"{g}_sample"] = np.asarray(k, np.float32).ravel(order="F")[::KSTRIDE].copy()
+            print(f"{g}: norm {out[f'{g}_norm']:.4e}", flush=True)
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    np.savez_compressed(FIXTURE, **out)
+    return FIXTURE
+
+
+if __name__ == "__main__":
+    print(main())
