@@ -265,13 +265,15 @@ def build_ref() -> bool:
     return r.returncode == 0 and ref_available()
 
 
-def build_ref_multirank(per_rank, grad_q: bool = False) -> str:
+def build_ref_multirank(per_rank, grad_q: bool = False, pml: Optional[int] = None) -> str:
     """Build the reference solver for runs whose rank boxes hold per_rank =
     (ex, ey, ez) elements each (its static extents nx_max, ny_max, nz_max set
     to ex-1, ey-1, ez-1, as the reference's workflow sets them per run; `make
     ref-mr`).  Returns the path of the solver, or with grad_q of the grad_Q
-    build (ses3d_grad_Q.f90 and the harness that writes grad_Q_mu)."""
-    tag = "_".join(str(int(v) - 1) for v in per_rank)
+    build (ses3d_grad_Q.f90 and the harness that writes grad_Q_mu).  ``pml``
+    also sets the build's absorbing width (SOURCE/ses3d_modules.f90:21,
+    default 2)."""
+    tag = "_".join(str(int(v) - 1) for v in per_rank) + (f"_{int(pml)}" if pml is not None else "")
     exe = os.path.join(HERE, "_ref", f"ses3d_{'grad_q_' if grad_q else ''}mr_{tag}.exe")
     r = subprocess.run(["make", "-s", "-C", HERE, "ref-mr", f"MR={tag}"], capture_output=True, text=True)
     if r.returncode != 0 or not os.path.exists(exe):

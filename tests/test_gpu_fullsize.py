@@ -3,7 +3,10 @@ points, 3 relaxation mechanisms, 3-element absorbing layers, 196 receivers).
 
 * Its first 20 steps (one captured 16-step graph and 4 direct steps) are
   compared with the float32 oracle bit for bit: velocity, split stresses,
-  memory variables and seismograms.
+  memory variables and seismograms; and with the reference solver itself,
+  run on one rank of a build sized for the section with pml=3
+  (tests/golden/make_golden_fullsize_ref.py): seismograms and velocity
+  snapshots bit for bit.
 * Over a longer run (300 steps) two size-independent properties:
   reproducibility -- a second run under a different launch schedule (4
   elements per workgroup, XCD order) gives the same seismograms bit for bit;
@@ -54,6 +57,30 @@ def test_fullsize_first_steps_bitexact():
     assert np.abs(seis_o).max() > 0
     _assert_same(s.seismograms(), seis_o, "seismograms")
     s.close()
+
+
+def test_fullsize_first_steps_match_reference():
+    import os
+    from ses3d_amd import configs
+    from tests.golden import make_golden_fullsize_ref as FR
+    from tests.golden.fixture_case import field_digest, input_digest
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "anelastic_96x96x48_ref.npz"))
+    w = configs.WORKLOADS[FR.WORKLOAD]
+    run = FR.build_run()
+    assert input_digest(run) == str(fx["digest"]), "inputs drifted from the fixture's"
+    nt = int(fx["nt"])
+    print(f"{w.name}: GPU {nt} steps against the reference", flush=True)
+    s = _solver(run, w)
+    s.run(1, nt - 1)
+    snap = {c: s.field(c) for c in ("vx", "vy", "vz")}
+    s.run(nt, 1)
+    seis = s.seismograms()
+    s.close()
+    assert np.abs(fx["seismograms"]).max() > 0
+    np.testing.assert_array_equal(seis, fx["seismograms"])
+    for c in ("vx", "vy", "vz"):
+        assert float(fx[f"{c}_norm"]) > 0, c
+        assert field_digest(snap[c]) == str(fx[f"{c}_digest"]), c
 
 
 def test_fullsize_reproducible_and_linear():
