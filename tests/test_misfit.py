@@ -44,3 +44,31 @@ def test_block_and_projection_files_round_trip(tmp_path):
     F.write_projected(str(tmp_path / "gradient_csh"), vals)
     got = F.read_projected(str(tmp_path / "gradient_csh"))
     assert np.array_equal(got[0], vals[0])
+
+
+def test_problem_measurement_hook():
+    """Problem(measure=fn) takes the event's misfit and adjoint rows from fn;
+    without it, the L2 waveform misfit and its rows."""
+    from ses3d_amd import configs
+    from ses3d_amd.autograd import Problem
+    w = configs.Workload("m", (2, 2, 2), False, 1, 0.5, 40, 3, "none")
+    run = configs.build(w, seed=0)
+    rng = np.random.default_rng(1)
+    u, d = rng.normal(size=(3, 3, 40)), rng.normal(size=(3, 3, 40))
+    rows = rng.normal(size=(3, 40, 3))
+    calls = []
+
+    def fn(ev, uu, dt):
+        calls.append((ev, dt))
+        return 2.5, rows
+
+    p = Problem(run, [], measure=fn)
+    chi, r = p._measure(0, u, True)
+    assert chi == 2.5 and calls == [(0, 0.5)]
+    assert r.dtype == np.float32 and r.flags.c_contiguous and np.array_equal(r, rows.astype(np.float32))
+    assert p._measure(0, u, False)[1] is None
+    q = Problem(run, [d], taper_width=5.0)
+    chi, r = q._measure(0, u, True)
+    d32 = d.astype(np.float32)          # observed data are held in float32
+    assert chi == MF.l2_misfit(u, d32, 0.5, q.taper)
+    assert np.array_equal(r, MF.l2_adjoint_rows(u, d32, 0.5, taper=q.taper))
