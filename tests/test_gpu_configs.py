@@ -5,10 +5,17 @@ oracle or HBM would not finish or fit), through size-independent properties.
   reference solver itself, run on one rank of a build sized for the section,
   and to the float32 oracle, which agrees with it bit for bit
   (tests/golden/elastic_24x24x12_oracle.npz, make_golden_config1.py).
-* config 3 (gradient on 48x48x24, anelastic): kernels bit-identical for
-  checkpoint intervals k = 50, 100, 200 and with every segment's sampled
-  steps kept (nothing replayed); the gradient against central finite
-  differences along 5 seeded random directions (h = 1e-3 relative).
+* config 3 (gradient on 48x48x24, anelastic): the checkpointed forward's
+  seismograms bit-identical to the reference solver's own (300 steps, one
+  rank of a build sized for the mesh, tests/golden/make_golden_gradient_ref.py);
+  with float32 accumulators the kernels (the four elastic ones and
+  grad_Q_mu) equal the reference's except at <= 0.35% of the sampled
+  points, where values are small (many in float32's subnormal range) and the
+  differences stay below 1e-8 of the kernel's norm (the float32 oracle
+  gives the same values as the GPU there), float64 accumulators within 1e-5; kernels bit-identical for checkpoint intervals k = 50, 100, 200 and
+  with every segment's sampled steps kept (nothing replayed); the gradient
+  against central finite differences along 5 seeded random directions
+  (h = 1e-3 relative).
 * config 4 (32 events on 64x64x32 plus 6 uneven sections in one group): each
   member after 30 grouped steps bit-identical to itself run alone.
 * config 5 (16 events on 64x64x32 through the torch.autograd.Function, in
@@ -69,6 +76,56 @@ def _gradient_case(nt):
     data = s.seismograms()
     s.close()
     return w, cur, data
+
+
+def test_config3_kernels_match_reference_solver(tmp_path):
+    from ses3d_amd import inputs as I, models as M
+    from tests.golden import make_golden_gradient_ref as GR
+    from tests.golden.fixture_case import field_digest, input_digest
+    fx = np.load(os.path.join(GOLDEN, "gradient_48x48x24_ref.npz"))
+    run = GR.build_run()
+    assert input_digest(run) == str(fx["digest"]), "inputs drifted from the fixture's"
+    w = configs.WORKLOADS[GR.WORKLOAD]
+    e = run.events[0]
+    root = str(tmp_path)
+    M.write_run_tree(root, M.RunCase(replace(run.setup, adjoint_flag=1), [e.event], [e.receivers], e.stf, run.model))
+    # the run tree as the reference read it: forward fields sampled every samp_ad = 15 steps
+    run1 = I.load_run(root)
+    assert run1.samp_ad == 15
+    print(f"config 3 against the reference: forward, {int(fx['nt'])} steps", flush=True)
+    f = _solver(run1, 1, w)
+    f.load_event(0, run1.events[0])
+    f.init()
+    f.forward_checkpointed(100)
+    np.testing.assert_array_equal(f.seismograms(), fx["seismograms"])
+    # the adjoint sources through the files' text, as the reference read them
+    GR.write_adjoint_files(root, run, fx["seismograms"])
+    run2 = I.load_run(root)
+    stride = int(fx["kstride"])
+    for bits in (32, 64):
+        print(f"adjoint, {bits}-bit kernel accumulators", flush=True)
+        a = _solver(run2, 2, w)
+        a.set_kernel_precision(bits)
+        a.load_event(0, run2.events[0])
+        a.init()
+        a.adjoint_checkpointed(f)
+        k = a.kernels(with_q=True)
+        a.close()
+        for g in GR.KERNELS:
+            assert float(fx[f"{g}_norm"]) > 0, g
+            got = np.asarray(k[g]).ravel(order="F")[::stride]
+            ref = fx[f"{g}_sample"].astype(np.float64)
+            rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            if bits == 32:
+                # the reference's float32 sums, except at the few points whose
+                # kernel values are small (differences below 1e-8 of the norm)
+                ndiff = int(np.sum(got != ref))
+                print(f"{g}: {ndiff} of {got.size} sampled values differ, rel {rel:.2e}", flush=True)
+                assert ndiff <= 0.005 * got.size and rel <= 1e-6, (g, ndiff, rel)
+                assert np.abs(got - ref).max() <= 1e-8 * float(fx[f"{g}_norm"]), g
+            else:
+                assert rel <= 1e-5, (g, rel)
+    f.close()
 
 
 def test_config3_kernels_same_for_every_checkpoint_interval():
