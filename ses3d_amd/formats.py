@@ -26,6 +26,7 @@ from __future__ import annotations
 
 import os
 from dataclasses import dataclass, field
+from fractions import Fraction
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -43,9 +44,30 @@ def _tok(line: str) -> str:
     return s.split()[0]
 
 
+def f32_array(tokens: Sequence[str]) -> np.ndarray:
+    """Parse tokens to float32 as a Fortran ``real`` read does: the decimal
+    value rounded once, to nearest float32.  Rounding to float64 first and
+    then to float32 gives the same result except where the float64 value lies
+    exactly halfway between two float32 values (the decimal can lie on either
+    side of it); those tokens are decided on their exact decimal value."""
+    toks = [t.strip().replace("d", "e").replace("D", "e") for t in tokens]
+    x = np.array([float(t) for t in toks], np.float64)
+    f = x.astype(np.float32)
+    with np.errstate(over="ignore", invalid="ignore"):
+        up = np.nextafter(f, np.float32(np.inf))
+        dn = np.nextafter(f, np.float32(-np.inf))
+        other = np.where(x > f.astype(np.float64), up, dn)
+        half = (f.astype(np.float64) + other.astype(np.float64)) * 0.5 == x
+    for i in np.flatnonzero(half & np.isfinite(f) & (x != f.astype(np.float64))):
+        exact, mid = Fraction(toks[i]), Fraction(float(x[i]))
+        if exact != mid and (exact > mid) == (other[i] > f[i]):
+            f[i] = other[i]
+    return f
+
+
 def f32(s: str) -> np.float32:
     """Parse one token to float32 (Fortran list-directed ``real`` read)."""
-    return np.float32(float(s.replace("d", "e").replace("D", "e")))
+    return f32_array([s])[0]
 
 
 def deg2rad32(v) -> np.float32:
@@ -664,7 +686,7 @@ def read_projected(path: str) -> List[np.ndarray]:
     n, k, out = int(toks[0]), 1, []
     for _ in range(n):
         m = int(toks[k]); k += 1
-        out.append(np.array([float(t) for t in toks[k:k + m]], np.float32))
+        out.append(f32_array(toks[k:k + m]))
         k += m
     return out
 
