@@ -296,7 +296,7 @@ def read_stf(path: str, nt: int) -> np.ndarray:
     samples, as in the reference."""
     L = _Lines(path)
     L.skip(4)
-    return np.array([f32(L.value()) for _ in range(nt)], np.float32)
+    return f32_array([L.value() for _ in range(nt)])
 
 
 def write_event_list(path: str, events: Sequence[str]) -> None:
@@ -629,15 +629,14 @@ def read_ad_src(path: str, nt: int) -> np.ndarray:
     4 header reads (blank records skipped), then nt rows."""
     L = _Lines(path)
     L.skip(4)
-    out = np.zeros((nt, 3), np.float32)
+    toks = []
     for i in range(nt):
         while True:
             line = L.next()
             if line.strip():
                 break
-        t = line.split()
-        out[i] = [f32(t[0]), f32(t[1]), f32(t[2])]
-    return out
+        toks += line.split()[:3]
+    return f32_array(toks).reshape(nt, 3)
 
 
 def write_saving_vector(path: str, sv: np.ndarray) -> None:
