@@ -114,6 +114,35 @@ def test_driver_on_rank_boxes(tmp_path):
             np.testing.assert_array_equal(k, run.box_part(fx[g], r), err_msg=f"{g} rank {r}")
 
 
+def test_driver_on_the_gradient_mesh(tmp_path):
+    """The gradient benchmark's mesh (48x48x24, anelastic, 3D model, 100
+    receivers, 300 steps, forward fields stored every 15 steps: 5 GB of
+    history files): seismogram files and, after the adjoint run, the four
+    kernel files bit-identical to the reference solver's
+    (tests/golden/make_golden_gradient_ref.py)."""
+    from dataclasses import replace
+    from ses3d_amd import models as M
+    from tests.golden import make_golden_gradient_ref as GR
+    from tests.golden.fixture_case import field_digest, input_digest
+    fx = np.load(os.path.join(GOLDEN, "gradient_48x48x24_ref.npz"))
+    run = GR.build_run()
+    assert input_digest(run) == str(fx["digest"]), "inputs drifted from the fixture's"
+    e = run.events[0]
+    root = str(tmp_path)
+    M.write_run_tree(root, M.RunCase(replace(run.setup, adjoint_flag=1), [e.event], [e.receivers], e.stf, run.model))
+    print("driver on the 48x48x24 gradient mesh: forward", flush=True)
+    _run(root)
+    ofd = os.path.join(root, "MAIN", e.event.ofd)
+    seis = np.stack([F.read_seismogram(ofd, n)["trace"].astype(np.float32) for n in e.receivers.names], 1)
+    np.testing.assert_array_equal(seis, fx["seismograms"])
+    GR.write_adjoint_files(root, run, fx["seismograms"])
+    print("adjoint", flush=True)
+    _run(root)
+    shape = (run.nx + 1, run.ny + 1, run.nz + 1, 5, 5, 5)
+    for g in ("grad_rho", "grad_cp", "grad_csh", "grad_csv"):
+        assert field_digest(F.read_field(os.path.join(ofd, f"{g}_0"), shape)) == str(fx[f"{g}_digest"]), g
+
+
 def test_driver_absorbing_width_at_run_time(tmp_path):
     """--pml 3 (the reference's compile-time pml, SOURCE/ses3d_modules.f90:21)
     gives the seismograms of the library run with pml=3, which differ from the
