@@ -114,6 +114,25 @@ def test_driver_on_rank_boxes(tmp_path):
             np.testing.assert_array_equal(k, run.box_part(fx[g], r), err_msg=f"{g} rank {r}")
 
 
+def test_driver_on_the_elastic_benchmark(tmp_path):
+    """The elastic 24x24x12 benchmark run tree (all 2000 steps): seismogram
+    files bit-identical to those of the reference solver run on one rank of a
+    build sized for the section (tests/golden/make_golden_config1.py)."""
+    from ses3d_amd import configs, models as M
+    from tests.golden.fixture_case import input_digest
+    fx = np.load(os.path.join(GOLDEN, "elastic_24x24x12_oracle.npz"))
+    run = configs.build(configs.WORKLOADS["elastic_24x24x12"], seed=0)
+    assert input_digest(run) == str(fx["digest"]), "inputs drifted from the fixture's"
+    e = run.events[0]
+    root = str(tmp_path)
+    M.write_run_tree(root, M.RunCase(run.setup, [e.event], [e.receivers], e.stf, run.model))
+    print("driver on the elastic 24x24x12 benchmark, 2000 steps", flush=True)
+    _run(root)
+    ofd = os.path.join(root, "MAIN", e.event.ofd)
+    seis = np.stack([F.read_seismogram(ofd, n)["trace"].astype(np.float32) for n in e.receivers.names], 1)
+    np.testing.assert_array_equal(seis, fx["ref_seismograms"])
+
+
 def test_driver_on_the_gradient_mesh(tmp_path):
     """The gradient benchmark's mesh (48x48x24, anelastic, 3D model, 100
     receivers, 300 steps, forward fields stored every 15 steps: 5 GB of
