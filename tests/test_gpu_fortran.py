@@ -133,6 +133,32 @@ def test_driver_on_the_elastic_benchmark(tmp_path):
     np.testing.assert_array_equal(seis, fx["ref_seismograms"])
 
 
+def test_driver_on_the_production_mesh(tmp_path):
+    """The 96x96x48 anelastic run tree (55.3 M GLL points, 200 receivers,
+    --pml 3), first 20 steps: seismogram and velocity snapshot files
+    bit-identical to the reference solver's (tests/golden/make_golden_fullsize_ref.py)."""
+    from dataclasses import replace
+    from ses3d_amd import models as M
+    from tests.golden import make_golden_fullsize_ref as FR
+    from tests.golden.fixture_case import field_digest, input_digest
+    fx = np.load(os.path.join(GOLDEN, "anelastic_96x96x48_ref.npz"))
+    run = FR.build_run()
+    assert input_digest(run) == str(fx["digest"]), "inputs drifted from the fixture's"
+    e = run.events[0]
+    nt = int(fx["nt"])
+    root = str(tmp_path)
+    ev = replace(e.event, output_displacement=1, ssamp=nt)
+    M.write_run_tree(root, M.RunCase(run.setup, [ev], [e.receivers], e.stf, run.model))
+    print(f"driver on the 96x96x48 section, {nt} steps", flush=True)
+    _run(root, "--pml", "3")
+    ofd = os.path.join(root, "MAIN", e.event.ofd)
+    seis = np.stack([F.read_seismogram(ofd, n)["trace"].astype(np.float32) for n in e.receivers.names], 1)
+    np.testing.assert_array_equal(seis, fx["seismograms"])
+    shape = (run.nx + 1, run.ny + 1, run.nz + 1, 5, 5, 5)
+    for c in ("vx", "vy", "vz"):
+        assert field_digest(F.read_field(os.path.join(ofd, f"{c}_0_{nt}"), shape)) == str(fx[f"{c}_digest"]), c
+
+
 def test_driver_on_the_gradient_mesh(tmp_path):
     """The gradient benchmark's mesh (48x48x24, anelastic, 3D model, 100
     receivers, 300 steps, forward fields stored every 15 steps: 5 GB of
