@@ -18,7 +18,7 @@ Workloads (spherical section colat 40-60 deg, lon 10-30 deg, r 5371-6371 km):
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import List, Optional
+from typing import Optional
 
 import numpy as np
 

@@ -1,9 +1,6 @@
 """Small seeded SES3D run trees for the tests (inputs in the reference formats)."""
 from __future__ import annotations
 
-import os
-import tempfile
-
 import numpy as np
 
 from ses3d_amd import formats as F, inputs as I, models as M

@@ -30,7 +30,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 
 from oracle import pyoracle as PO  # noqa: E402
-from ses3d_amd import configs, formats as F, inputs as I, models as M  # noqa: E402
+from ses3d_amd import configs, formats as F, models as M  # noqa: E402
 from tests.golden.fixture_case import field_digest, input_digest  # noqa: E402
 
 FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gradient_48x48x24_ref.npz")

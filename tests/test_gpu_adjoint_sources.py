@@ -18,7 +18,6 @@ the tapers are nearly zero there, which the bound for them allows for.
 import numpy as np
 import pytest
 import torch
-from dataclasses import replace
 
 from ses3d_amd import adjoint_source as A, configs, models as M
 

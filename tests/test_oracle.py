@@ -10,7 +10,6 @@
    one to float32 rounding.
 """
 import os
-import tempfile
 
 import numpy as np
 import pytest
